@@ -1,0 +1,371 @@
+"""Python host side of the MI355X VVM timestep.
+
+Binds the C ABI of ``libvvm_amd.so`` (declared in ``include/vvm_dev.h``) with
+ctypes, owns host copies of the model state in the reference's ``[i][k]``
+layout (numpy arrays), reads and writes the binary state files the reference
+harness uses, and defines the benchmark cases.
+
+The compute path is the HIP library: nothing here computes a timestep, and
+``Device`` raises if the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "libvvm_amd.so")
+
+VVM_SOLVER_FP64, VVM_SOLVER_MIXED = 0, 1
+VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED = 0, 1, 2, 3
+VVM_STOP_PARITY, VVM_STOP_REL = 0, 1
+
+OPS = {
+    "pzeta_pt": 1, "pth_pt": 2, "pqv_pt": 3, "pqc_pt": 4, "pqr_pt": 5, "forcing_bc": 6,
+    "pubarTop_pt": 7, "cal_w": 8, "cal_u": 9, "microphysics": 10, "turbulence": 11,
+    "nudge": 12, "nextTimeStep": 13, "bc_all": 14,
+}
+
+
+class VvmDevConfig(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("nz", C.c_int32),
+        ("dx", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("GRAVITY", C.c_double), ("Cp", C.c_double), ("Cv", C.c_double), ("Rd", C.c_double),
+        ("Lv", C.c_double), ("P0", C.c_double), ("PSURF", C.c_double),
+        ("addforcingtime", C.c_double), ("moisture_nudge_time", C.c_double), ("tolerance", C.c_double),
+        ("CASE", C.c_int32), ("tropical_forcing", C.c_int32), ("solver_mode", C.c_int32),
+        ("guess_mode", C.c_int32), ("stop_mode", C.c_int32), ("max_iter", C.c_int32),
+        ("tol_rel", C.c_double),
+    ]
+
+
+FIELDS2D = ["zeta", "zetam", "zetap", "th", "thm", "thp", "qv", "qvm", "qvp", "qc", "qcm", "qcp",
+            "qr", "qrm", "qrp", "u", "w", "RKM", "RKH", "init_th_forcing",
+            "evaporation", "accretion", "autoconversion", "condensation"]
+HISTS = ["dth_advect", "dth_buoyancy", "dzeta_advect", "dqv_advect", "dqc_advect", "dqr_advect", "dqr_VT"]
+ROWS = ["precip", "xi", "uxi"]
+PROFILES = ["rhou", "rhow", "pib", "pb", "thb", "thbm", "thb_init", "thb_zeta", "thvb", "thvbm",
+            "qvb", "qvb0", "qvsb", "z", "z_zeta", "lambda2", "Q1LS", "Q2LS"]
+SCALARS = ["ubarTop", "ubarTopp", "ubarTopm"]
+PTR = C.POINTER(C.c_double)
+
+
+class VvmHostView(C.Structure):
+    _fields_ = ([(n, PTR) for n in FIELDS2D + HISTS + ROWS + PROFILES]
+                + [(n, C.c_double) for n in SCALARS]
+                + [("dubarTop_advect", C.c_double * 2), ("step", C.c_int32),
+                   ("status_for_adding_forcing", C.c_int32)])
+
+
+class VvmStepStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("solve_iters_total", C.c_int32), ("solve_iters_last", C.c_int32),
+                ("refine_last", C.c_int32), ("not_converged", C.c_int32), ("rel_residual_last", C.c_double),
+                ("t_advection", C.c_double), ("t_poisson", C.c_double), ("t_microphysics", C.c_double),
+                ("t_diffusion", C.c_double), ("t_all", C.c_double)]
+
+
+class VvmSolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("refine", C.c_int32), ("rel_residual", C.c_double),
+                ("rel_residual0", C.c_double), ("seconds", C.c_double)]
+
+
+class VvmCaseParams(C.Structure):
+    _fields_ = [("sounding", C.c_int32), ("brunt", C.c_double), ("bubble_dT", C.c_double),
+                ("bubble_radius", C.c_double), ("bubble_zc", C.c_double), ("noise_kind", C.c_int32),
+                ("noise_amp", C.c_double), ("noise_ztop", C.c_double), ("seed", C.c_uint64),
+                ("sfc_heat_flux", C.c_double), ("sfc_qv_ramp", C.c_double), ("sfc_qv_ramp_z", C.c_double)]
+
+
+_LIB = None
+
+
+def load_lib(path: str = LIB_PATH):
+    """Load the HIP library; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it with __graft_entry__.build()")
+    lib = C.CDLL(path)
+    P = C.POINTER
+    sig = {
+        "vvm_dev_create": (C.c_int, [P(VvmDevConfig), C.c_int, P(C.c_void_p)]),
+        "vvm_dev_destroy": (None, [C.c_void_p]),
+        "vvm_dev_last_error": (C.c_char_p, [C.c_void_p]),
+        "vvm_dev_upload": (C.c_int, [C.c_void_p, P(VvmHostView)]),
+        "vvm_dev_download": (C.c_int, [C.c_void_p, P(VvmHostView)]),
+        "vvm_dev_step": (C.c_int, [C.c_void_p, C.c_int, P(VvmStepStats)]),
+        "vvm_dev_poisson": (C.c_int, [C.c_void_p, PTR, PTR, P(VvmSolveStats)]),
+        "vvm_dev_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, P(VvmSolveStats)]),
+        "vvm_dev_op": (C.c_int, [C.c_void_p, C.c_int]),
+        "vvm_init_case": (C.c_int, [P(VvmDevConfig), P(VvmCaseParams), P(VvmHostView)]),
+        "vvm_random_perturbation": (C.c_int, [P(VvmDevConfig), P(VvmHostView), C.c_int,
+                                              C.c_double, C.c_double, C.c_double]),
+        "vvm_field_len": (C.c_int64, [C.c_int, C.c_int]),
+        "vvm_hist_len": (C.c_int64, [C.c_int, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name, None)
+        if fn is None:
+            raise RuntimeError(f"{path} does not export {name}")
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+# ----------------------------------------------------------------- config
+
+CONSTANTS = dict(GRAVITY=9.80665, Cp=1003.5, Cv=716.5, Rd=287.0, Lv=2.5e6, P0=1e5, PSURF=96500.0,
+                 tolerance=1e-22)
+
+
+def make_config(nx_int: int, nz_int: int, dx: float, dz: float, dt: float, *, CASE: int = 1,
+                addforcingtime: float = 0.0, moisture_nudge_time: float = 0.0, tropical_forcing: int = 1,
+                solver_mode: int = VVM_SOLVER_FP64, guess_mode: int = VVM_GUESS_ZERO,
+                stop_mode: int = VVM_STOP_PARITY, max_iter: int = 200, tol_rel: float = 1e-10) -> VvmDevConfig:
+    """Config with the constants main() passes to Config_VVM (src/main.cpp:68-72)."""
+    return VvmDevConfig(nx=nx_int + 2, nz=nz_int + 2, dx=dx, dz=dz, dt=dt, addforcingtime=addforcingtime,
+                        moisture_nudge_time=moisture_nudge_time, CASE=CASE, tropical_forcing=tropical_forcing,
+                        solver_mode=solver_mode, guess_mode=guess_mode, stop_mode=stop_mode,
+                        max_iter=max_iter, tol_rel=tol_rel, **CONSTANTS)
+
+
+@dataclass
+class Case:
+    name: str
+    cfg: VvmDevConfig
+    params: VvmCaseParams
+    steps: int
+
+
+def case(name: str, seed: int = 1, **over) -> Case:
+    """Benchmark cases.  c0: 128x64 warm bubble; c1: 1024x128 with surface
+    fluxes; c2: 2048x256 with the c1 forcing; tiny: 16x12 for operator tests."""
+    sfc = dict(sfc_heat_flux=100.0, sfc_qv_ramp=5e-3, sfc_qv_ramp_z=500.0)
+    if name == "c0":
+        cfg = make_config(128, 64, 200.0, 200.0, 2.0, **over)
+        p = VvmCaseParams(sounding=0, brunt=0.01, bubble_dT=2.0, bubble_radius=2000.0, bubble_zc=2000.0,
+                          noise_kind=1, noise_amp=0.01, noise_ztop=1000.0, seed=seed)
+        return Case(name, cfg, p, 600)
+    if name in ("c1", "c2"):
+        nxi, nzi, steps = (1024, 128, 1000) if name == "c1" else (2048, 256, 500)
+        cfg = make_config(nxi, nzi, 100.0, 100.0, 1.0, **over)
+        p = VvmCaseParams(sounding=0, brunt=0.01, noise_kind=2, noise_amp=0.1, noise_ztop=500.0, seed=seed, **sfc)
+        return Case(name, cfg, p, steps)
+    if name == "tiny":
+        cfg = make_config(16, 12, 200.0, 200.0, 2.0, **over)
+        p = VvmCaseParams(sounding=0, brunt=0.01, bubble_dT=2.0, bubble_radius=800.0, bubble_zc=600.0,
+                          noise_kind=1, noise_amp=0.05, noise_ztop=1000.0, seed=seed,
+                          sfc_heat_flux=100.0, sfc_qv_ramp=5e-3, sfc_qv_ramp_z=500.0)
+        return Case(name, cfg, p, 10)
+    raise KeyError(name)
+
+
+def grid_case(nx_int: int, nz_int: int, dx: float = 100.0, dt: float = 1.0, seed: int = 1,
+              sigma: float = 0.1, heat: float = 100.0, **over) -> Case:
+    """A c1-style case on any grid (ensemble members, parity at odd sizes)."""
+    cfg = make_config(nx_int, nz_int, dx, dx, dt, **over)
+    p = VvmCaseParams(sounding=0, brunt=0.01, noise_kind=2, noise_amp=sigma, noise_ztop=500.0, seed=seed,
+                      sfc_heat_flux=heat, sfc_qv_ramp=5e-3, sfc_qv_ramp_z=500.0)
+    return Case(f"{nx_int}x{nz_int}", cfg, p, 300)
+
+
+# ------------------------------------------------------------- host state
+
+class HostState:
+    """Host copy of the model state: 2-D arrays shaped (nx, nz) ([i][k]),
+    histories (nx, nz, 2), rows (nx,), profiles (nz,)."""
+
+    def __init__(self, nx: int, nz: int):
+        self.nx, self.nz = nx, nz
+        self.a: dict[str, np.ndarray] = {}
+        for n in FIELDS2D:
+            self.a[n] = np.zeros((nx, nz))
+        for n in HISTS:
+            self.a[n] = np.zeros((nx, nz, 2))
+        for n in ROWS:
+            self.a[n] = np.zeros(nx)
+        for n in PROFILES:
+            self.a[n] = np.zeros(nz)
+        self.s = {n: 0.0 for n in SCALARS}
+        self.dubarTop_advect = [0.0, 0.0]
+        self.step = 0
+        self.status_for_adding_forcing = 0
+
+    def __getitem__(self, name):
+        return self.a[name]
+
+    def copy(self) -> "HostState":
+        h = HostState(self.nx, self.nz)
+        for n, arr in self.a.items():
+            h.a[n][...] = arr
+        h.s = dict(self.s)
+        h.dubarTop_advect = list(self.dubarTop_advect)
+        h.step = self.step
+        h.status_for_adding_forcing = self.status_for_adding_forcing
+        return h
+
+    def view(self) -> VvmHostView:
+        v = VvmHostView()
+        for n, arr in self.a.items():
+            assert arr.flags.c_contiguous and arr.dtype == np.float64
+            setattr(v, n, arr.ctypes.data_as(PTR))
+        for n in SCALARS:
+            setattr(v, n, self.s[n])
+        v.dubarTop_advect[0], v.dubarTop_advect[1] = self.dubarTop_advect
+        v.step = self.step
+        v.status_for_adding_forcing = self.status_for_adding_forcing
+        return v
+
+    def pull_scalars(self, v: VvmHostView) -> None:
+        for n in SCALARS:
+            self.s[n] = getattr(v, n)
+        self.dubarTop_advect = [v.dubarTop_advect[0], v.dubarTop_advect[1]]
+        self.step = v.step
+        self.status_for_adding_forcing = v.status_for_adding_forcing
+
+    # -------- state files (format of oracle/ref/ref_harness.cpp)
+    def save(self, path: str, cfg: VvmDevConfig) -> None:
+        recs = []
+        for n in ("nx", "nz", "dx", "dz", "dt", "GRAVITY", "Cp", "Cv", "Rd", "Lv", "P0", "PSURF",
+                  "addforcingtime", "moisture_nudge_time", "tolerance", "CASE"):
+            recs.append((f"cfg.{n}", np.array([float(getattr(cfg, n))])))
+        for n, arr in self.a.items():
+            recs.append((n, arr))
+        for n in SCALARS:
+            recs.append((n, np.array([self.s[n]])))
+        recs.append(("dubarTop_advect0", np.array([self.dubarTop_advect[0]])))
+        recs.append(("dubarTop_advect1", np.array([self.dubarTop_advect[1]])))
+        recs.append(("step", np.array([float(self.step)])))
+        recs.append(("status_for_adding_forcing", np.array([float(self.status_for_adding_forcing)])))
+        with open(path, "wb") as f:
+            f.write(b"VVMSTATE" + struct.pack("<ii", 1, len(recs)))
+            for name, arr in recs:
+                dims = list(arr.shape) + [1] * (3 - arr.ndim)
+                f.write(name.encode().ljust(32, b"\0") + struct.pack("<iiii", 3, *dims))
+                f.write(np.ascontiguousarray(arr, dtype="<f8").tobytes())
+
+    @staticmethod
+    def read_records(path: str) -> dict[str, np.ndarray]:
+        out = {}
+        with open(path, "rb") as f:
+            data = f.read()
+        if data[:8] != b"VVMSTATE":
+            raise ValueError("bad state file")
+        _, nrec = struct.unpack_from("<ii", data, 8)
+        off = 16
+        for _ in range(nrec):
+            name = data[off:off + 32].split(b"\0")[0].decode()
+            _, d0, d1, d2 = struct.unpack_from("<iiii", data, off + 32)
+            off += 48
+            n = d0 * d1 * d2
+            arr = np.frombuffer(data, dtype="<f8", count=n, offset=off).copy()
+            off += 8 * n
+            shape = [d for d in (d0, d1, d2)]
+            while len(shape) > 1 and shape[-1] == 1:
+                shape.pop()
+            out[name] = arr.reshape(shape)
+        return out
+
+    @classmethod
+    def load(cls, path: str) -> "HostState":
+        r = cls.read_records(path)
+        nx, nz = r["zeta"].shape
+        h = cls(nx, nz)
+        for n in h.a:
+            if n in r:
+                h.a[n][...] = r[n].reshape(h.a[n].shape)
+        for n in SCALARS:
+            h.s[n] = float(r[n][0])
+        h.dubarTop_advect = [float(r["dubarTop_advect0"][0]), float(r["dubarTop_advect1"][0])]
+        h.step = int(r["step"][0])
+        h.status_for_adding_forcing = int(r["status_for_adding_forcing"][0])
+        return h
+
+
+def init_state(c: Case) -> HostState:
+    """Initial state of a case, from the library's host init layer."""
+    lib = load_lib()
+    h = HostState(c.cfg.nx, c.cfg.nz)
+    v = h.view()
+    rc = lib.vvm_init_case(C.byref(c.cfg), C.byref(c.params), C.byref(v))
+    if rc != 0:
+        raise RuntimeError(f"vvm_init_case failed: {rc}")
+    h.pull_scalars(v)
+    return h
+
+
+# ---------------------------------------------------------------- device
+
+class VvmError(RuntimeError):
+    pass
+
+
+class Device:
+    """One device handle (one GPU, one stream)."""
+
+    def __init__(self, cfg: VvmDevConfig, device: int = 0):
+        self.lib = load_lib()
+        self.cfg = cfg
+        self.h = C.c_void_p()
+        rc = self.lib.vvm_dev_create(C.byref(cfg), device, C.byref(self.h))
+        if rc != 0:
+            raise VvmError(f"vvm_dev_create failed ({rc})")
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise VvmError(f"{what} failed ({rc}): {self.lib.vvm_dev_last_error(self.h).decode()}")
+        return rc
+
+    def upload(self, st: HostState) -> None:
+        v = st.view()
+        self._check(self.lib.vvm_dev_upload(self.h, C.byref(v)), "upload")
+
+    def download(self, st: HostState) -> None:
+        v = st.view()
+        self._check(self.lib.vvm_dev_download(self.h, C.byref(v)), "download")
+        st.pull_scalars(v)
+
+    def step(self, n: int = 1) -> VvmStepStats:
+        s = VvmStepStats()
+        self._check(self.lib.vvm_dev_step(self.h, n, C.byref(s)), "step")
+        return s
+
+    def op(self, name: str) -> int:
+        return self._check(self.lib.vvm_dev_op(self.h, OPS[name]), f"op {name}")
+
+    def poisson(self, b: np.ndarray, x: np.ndarray) -> VvmSolveStats:
+        s = VvmSolveStats()
+        assert b.dtype == np.float64 and x.dtype == np.float64 and b.flags.c_contiguous and x.flags.c_contiguous
+        self._check(self.lib.vvm_dev_poisson(self.h, b.ctypes.data_as(PTR), x.ctypes.data_as(PTR), C.byref(s)),
+                    "poisson")
+        return s
+
+    def poisson_device(self, d_b: int, d_x: int) -> VvmSolveStats:
+        s = VvmSolveStats()
+        self._check(self.lib.vvm_dev_poisson_device(self.h, C.c_void_p(d_b), C.c_void_p(d_x), C.byref(s)),
+                    "poisson_device")
+        return s
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.vvm_dev_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

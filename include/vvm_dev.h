@@ -1,0 +1,203 @@
+/*
+ * vvm_dev.h -- C ABI of the MI355X dynamical-core timestep of the 2D vector
+ * vorticity model (VVM).
+ *
+ * The reference swaps its elliptic back-end at compile time behind the
+ * static members of class vvm (src/Declare.hpp:707-941; the GPU_POISSON
+ * variant src/PoissonSolver_GPU_mpi.cpp keeps the same cal_w / cal_u /
+ * pubarTop_pt / InitPoissonMatrix signatures).  This header is the thin C
+ * layer below a Config_VVM / vvm compatible C++ API: plain pointers and
+ * sizes, no C++ types, no exceptions, no exit().
+ *
+ * Layout conventions
+ *   - nx, nz are the array extents INCLUDING one ghost cell on each side,
+ *     exactly as the reference's vvm::nx / vvm::nz (src/Declare.hpp:82-83).
+ *   - Host 2-D fields are the reference's contiguous blocks `<name>cont`
+ *     (src/Declare.hpp:400-407): element (i,k) at [i*nz + k], k fastest.
+ *   - Host AB2 histories are the reference's `***` blocks
+ *     (src/Declare.hpp:417-427): element (i,k,slot) at [(i*nz + k)*2 + slot].
+ *   - The device keeps every 2-D field x-contiguous ([k][i], padded pitch);
+ *     upload / download transpose once, never per step.
+ *
+ * Return codes: 0 success, > 0 warning (solver did not reach its stop rule;
+ * the step continued, as the reference does at src/PoissonSolver.cpp:463-467),
+ * < 0 error (see vvm_dev_last_error).
+ */
+#ifndef VVM_DEV_H
+#define VVM_DEV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VVM_OK 0
+#define VVM_WARN_NOT_CONVERGED 1
+#define VVM_ERR_ARG (-1)
+#define VVM_ERR_HIP (-2)
+#define VVM_ERR_SIZE (-3)
+#define VVM_ERR_STATE (-4)
+#define VVM_ERR_NOMEM (-5)
+
+/* Poisson back-ends (north of the reference's single Eigen path). */
+#define VVM_SOLVER_FP64 0   /* FP64 multigrid-preconditioned CG            */
+#define VVM_SOLVER_MIXED 1  /* FP32 MG-PCG inside FP64 iterative refinement */
+
+/* Initial guess for the cal_w unknown x = rhow[k]*w[i][k]. */
+#define VVM_GUESS_ZERO 0     /* reference behaviour (Eigen x.setZero())     */
+#define VVM_GUESS_PREV 1     /* x from the previous step                    */
+#define VVM_GUESS_EXTRAP 2   /* 2 x_{t-1} - x_{t-2}                         */
+#define VVM_GUESS_LEARNED 3  /* conv-net correction on top of x_{t-1}       */
+
+/* Stop rules of the elliptic solve. */
+#define VVM_STOP_PARITY 0  /* iterate until the residual stops decreasing   */
+#define VVM_STOP_REL 1     /* ||b - A x||_2 / ||b||_2 <= tol_rel            */
+
+/* Model configuration: the runtime members of the reference's Config_VVM
+ * (src/Declare.hpp:27-70) that the timestep reads, plus back-end options. */
+typedef struct vvm_dev_config {
+    int32_t nx, nz;          /* extents including ghosts                     */
+    double dx, dz, dt;
+    double GRAVITY, Cp, Cv, Rd, Lv, P0, PSURF;
+    double addforcingtime;   /* s; random theta kick while step*dt <= this   */
+    double moisture_nudge_time; /* s; 0 disables Nudge_qv                    */
+    double tolerance;        /* reference relative tolerance (1e-22)         */
+    int32_t CASE;            /* CASE==2 skips Nudge_zeta                     */
+    int32_t tropical_forcing;/* 1: thp += dt*Q1LS[k], qvp += dt*Q2LS[k]      */
+    int32_t solver_mode;     /* VVM_SOLVER_*                                 */
+    int32_t guess_mode;      /* VVM_GUESS_*                                  */
+    int32_t stop_mode;       /* VVM_STOP_*                                   */
+    int32_t max_iter;        /* cap on PCG iterations per solve              */
+    double tol_rel;          /* used by VVM_STOP_REL                         */
+} vvm_dev_config;
+
+/* Host view of the whole model state: the `...cont` blocks and 1-D profiles
+ * of the reference's class vvm (src/Declare.hpp:490-699).  Any pointer may
+ * be NULL on download (that array is skipped); on upload every pointer
+ * listed as "state" must be set. */
+typedef struct vvm_host_view {
+    /* 2-D prognostic triplets (state) */
+    double *zeta, *zetam, *zetap;
+    double *th, *thm, *thp;
+    double *qv, *qvm, *qvp;
+    double *qc, *qcm, *qcp;
+    double *qr, *qrm, *qrp;
+    /* 2-D diagnostics (state: u, w, RKM, RKH; the rest outputs only) */
+    double *u, *w, *RKM, *RKH, *init_th_forcing;
+    double *evaporation, *accretion, *autoconversion, *condensation;
+    /* AB2 histories [i][k][2] (state) */
+    double *dth_advect, *dth_buoyancy, *dzeta_advect;
+    double *dqv_advect, *dqc_advect, *dqr_advect, *dqr_VT;
+    /* nx-long rows */
+    double *precip, *xi, *uxi;
+    /* nz-long profiles (state) */
+    double *rhou, *rhow, *pib, *pb, *thb, *thbm, *thb_init, *thb_zeta;
+    double *thvb, *thvbm, *qvb, *qvb0, *qvsb, *z, *z_zeta, *lambda2;
+    double *Q1LS, *Q2LS;
+    /* scalars */
+    double ubarTop, ubarTopp, ubarTopm;
+    double dubarTop_advect[2];
+    int32_t step;
+    int32_t status_for_adding_forcing;
+} vvm_host_view;
+
+/* Per-call statistics; per-phase times mirror the reference's t_* arrays
+ * (src/Declare.hpp:695-699, filled at src/Iteration.cpp:286,297,405,418,435). */
+typedef struct vvm_step_stats {
+    int32_t steps;             /* steps taken by this call                   */
+    int32_t solve_iters_total; /* PCG iterations summed over the steps      */
+    int32_t solve_iters_last;  /* PCG iterations of the last step           */
+    int32_t refine_last;       /* outer refinement sweeps of the last step  */
+    int32_t not_converged;     /* solves that hit max_iter                   */
+    double rel_residual_last;  /* ||b-Ax||/||b|| of the last solve          */
+    double t_advection, t_poisson, t_microphysics, t_diffusion, t_all; /* s */
+} vvm_step_stats;
+
+typedef struct vvm_solve_stats {
+    int32_t iters;          /* PCG iterations (inner, summed)               */
+    int32_t refine;         /* outer FP64 refinement sweeps (mixed only)    */
+    double rel_residual;    /* ||b - A x||_2 / ||b||_2                      */
+    double rel_residual0;   /* same, for the initial guess                  */
+    double seconds;         /* device time of the solve                     */
+} vvm_solve_stats;
+
+typedef struct vvm_dev vvm_dev_t;
+
+/* Create a handle bound to one device and one stream; allocates all device
+ * memory for the grid.  Replaces the back-end set-up the reference does in
+ * InitPoissonMatrix (src/PoissonSolver.cpp:681-813) / InitAMGX
+ * (src/PoissonSolver_GPU_mpi.cpp:8-80). */
+int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out);
+void vvm_dev_destroy(vvm_dev_t *h);
+const char *vvm_dev_last_error(const vvm_dev_t *h);
+
+/* Copy the full host state to the device ([i][k] -> [k][i]). */
+int vvm_dev_upload(vvm_dev_t *h, const vvm_host_view *v);
+/* Copy the device state back to the host (non-NULL pointers only). */
+int vvm_dev_download(vvm_dev_t *h, vvm_host_view *v);
+
+/* Advance nsteps steps of vvm::Iteration::TimeMarching's loop body
+ * (src/Iteration.cpp:227-436) on the device; returns after the stream
+ * synchronises.  stats may be NULL. */
+int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats);
+
+/* Isolated elliptic solve of the reference's cal_w system A x = b
+ * (src/PoissonSolver.cpp:441-467, matrix src/PoissonSolver.cpp:760-796):
+ * b and x are host arrays of (nx-2)*(nz-3) doubles in the reference's row
+ * order idx = (k-2)*(nx-2) + (i-1).  x is the initial guess on entry. */
+int vvm_dev_poisson(vvm_dev_t *h, const double *b, double *x, vvm_solve_stats *stats);
+/* Same with device-resident b and x (same layout), for benches. */
+int vvm_dev_poisson_device(vvm_dev_t *h, const double *d_b, double *d_x, vvm_solve_stats *stats);
+
+/* Per-operator parity hooks: run one reference operator on the device
+ * state (tests).  op is one of VVM_OP_*. */
+#define VVM_OP_PZETA 1        /* Iteration::pzeta_pt   src/Iteration.cpp:29-34   */
+#define VVM_OP_PTH 2          /* Iteration::pth_pt     src/Iteration.cpp:36-40   */
+#define VVM_OP_PQV 3          /* Iteration::pqv_pt     src/Iteration.cpp:90-98   */
+#define VVM_OP_PQC 4          /* Iteration::pqc_pt     src/Iteration.cpp:100-107 */
+#define VVM_OP_PQR 5          /* Iteration::pqr_pt     src/Iteration.cpp:109-124 */
+#define VVM_OP_FORCING_BC 6   /* AddForcing + BoundaryProcess2D_all (:283-285)  */
+#define VVM_OP_PUBARTOP 7     /* PoissonSolver::pubarTop_pt :655-676            */
+#define VVM_OP_CAL_W 8        /* PoissonSolver::cal_w :439-477                  */
+#define VVM_OP_CAL_U 9        /* PoissonSolver::cal_u :580-653                  */
+#define VVM_OP_MICROPHYSICS 10/* Kessler + NegativeValueProcess x3 + BC :302-312,404 */
+#define VVM_OP_TURBULENCE 11  /* Turbulence::RKM_RKH (+Mparam/Hparam) Turbulent.cpp:5-66 */
+#define VVM_OP_NUDGE 12       /* Nudge_theta/zeta/qv NumericalProcess.cpp:60-96 */
+#define VVM_OP_NEXT 13        /* Iteration::nextTimeStep :153-204               */
+#define VVM_OP_BC_ALL 14      /* BoundaryProcess2D_all Boundary.cpp:34-52       */
+int vvm_dev_op(vvm_dev_t *h, int op);
+
+/* Host-side helpers (no GPU use). */
+
+/* Fill the analytic base state and initial fields of one of the benchmark
+ * cases (see vvm_case_params) into a caller-allocated host view. */
+typedef struct vvm_case_params {
+    int32_t sounding;        /* 0: theta = 300 K exp(N^2 z/g), RH ramp       */
+    double brunt;            /* N [1/s]                                       */
+    double bubble_dT;        /* K, cosine bubble amplitude (0: none)          */
+    double bubble_radius;    /* m                                             */
+    double bubble_zc;        /* m                                             */
+    int32_t noise_kind;      /* 0: none, 1: uniform +-amp, 2: gaussian sigma  */
+    double noise_amp;        /* K                                             */
+    double noise_ztop;       /* m; noise only below this height               */
+    uint64_t seed;
+    double sfc_heat_flux;    /* W/m^2 -> Q1LS at k=1                          */
+    double sfc_qv_ramp;      /* kg/kg/day at the surface, ramped to 0 at ramp_z */
+    double sfc_qv_ramp_z;    /* m                                             */
+} vvm_case_params;
+int vvm_init_case(const vvm_dev_config *cfg, const vvm_case_params *p, vvm_host_view *v);
+
+/* The reference's Init::RandomPerturbation (src/Init.cpp:488-515): fills
+ * init_th_forcing from std::mt19937(t) and std::normal_distribution. */
+int vvm_random_perturbation(const vvm_dev_config *cfg, vvm_host_view *v, int t,
+                            double min_range, double max_range, double stddev);
+
+/* Number of doubles a host view needs for each class of array. */
+int64_t vvm_field_len(int nx, int nz);   /* nx*nz   */
+int64_t vvm_hist_len(int nx, int nz);    /* nx*nz*2 */
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+"""ctypes binding of the CPU oracle (oracle/vvm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker.  Only tests/, __graft_entry__.smoke()
+and bench.py's cpu_baseline leg may import this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "vvm_ref")
+
+GSOLVE_CG, GSOLVE_EXACT = 0, 1
+_LIB = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _LIB = C.CDLL(LIB_PATH)
+    return _LIB
+
+
+def _call(name, cfg, st, *extra):
+    lib = load()
+    v = st.view()
+    rc = getattr(lib, name)(C.byref(cfg), C.byref(v), *extra)
+    st.pull_scalars(v)
+    if rc != 0:
+        raise RuntimeError(f"{name} returned {rc}")
+
+
+def step(cfg, st, nsteps: int = 1, gsolve: int = GSOLVE_EXACT) -> None:
+    """nsteps TimeMarching loop bodies in place on a vvm_amd.HostState."""
+    _call("vo_step", cfg, st, C.c_int(nsteps), C.c_int(gsolve))
+
+
+OPS = {
+    "pzeta_pt": "vo_pzeta_pt", "pth_pt": "vo_pth_pt", "pqv_pt": "vo_pqv_pt", "pqc_pt": "vo_pqc_pt",
+    "pqr_pt": "vo_pqr_pt", "forcing_bc": "vo_forcing_bc", "bc_all": "vo_bc_all",
+    "pubarTop_pt": "vo_pubartop_pt", "cal_w": "vo_cal_w", "microphysics": "vo_microphysics",
+    "turbulence": "vo_turbulence", "nudge": "vo_nudge", "updateMean": "vo_update_mean",
+    "nextTimeStep": "vo_next_time_step",
+}
+
+
+def op(name: str, cfg, st, gsolve: int = GSOLVE_EXACT) -> None:
+    if name == "cal_u":
+        _call("vo_cal_u", cfg, st, C.c_int(gsolve))
+    else:
+        _call(OPS[name], cfg, st)
+
+
+def poisson_exact(cfg, rhou, rhow, b):
+    import numpy as np
+    lib = load()
+    x = np.zeros_like(b)
+    P = C.POINTER(C.c_double)
+    rc = lib.vo_poisson_exact(C.byref(cfg), rhou.ctypes.data_as(P), rhow.ctypes.data_as(P),
+                              b.ctypes.data_as(P), x.ctypes.data_as(P))
+    if rc != 0:
+        raise RuntimeError("vo_poisson_exact failed")
+    return x
+
+
+def poisson_apply(cfg, rhou, rhow, x):
+    import numpy as np
+    lib = load()
+    y = np.zeros_like(x)
+    P = C.POINTER(C.c_double)
+    lib.vo_poisson_apply(C.byref(cfg), rhou.ctypes.data_as(P), rhow.ctypes.data_as(P),
+                         x.ctypes.data_as(P), y.ctypes.data_as(P))
+    return y
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_BIN)
+
+
+def run_ref(mode: str, arg: str, in_path: str, out_path: str, timeout: float = 600) -> None:
+    """Run the compiled reference harness (oracle/_ref/vvm_ref), single thread."""
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([REF_BIN, mode, str(arg), in_path, out_path], check=True, env=env, timeout=timeout)

@@ -1,0 +1,65 @@
+#!/usr/bin/env bash
+# Builds the reference model's timestep from its own sources where they lie
+# under $VVM_REFERENCE (default /root/reference) into oracle/_ref/vvm_ref,
+# driven by ref_harness.cpp.  Used only as the checker and as the CPU
+# baseline; the product never links or calls it.
+#
+# The reference is configured by editing src/Config.hpp (its own comment:
+# "Here are the parameters for the model that you can tune").  Its default
+# selects P3 + RRTMGP + netCDF output, which need Fortran, Boost and netCDF
+# that this image lacks.  The recipe therefore stages the needed files in a
+# scratch directory inside oracle/_ref/ (deleted on exit, never committed,
+# never shipped) and applies exactly these edits:
+#   1. Config.hpp: keep AB2, WATER, TROPICALFORCING; turn on KESSLER_MICROPHY;
+#      turn off OUTPUTNC, OUTPUTNCSAMEFILE, P3_MICROPHY, RTERRTMGP, LOADFILE.
+#   2. AddForcing.cpp: drop the radiation_heating_rate line (the member only
+#      exists under RTERRTMGP, src/Declare.hpp:298).
+#   3. Init.cpp: drop the pb_lev_all loop in Init2d (P3-only member,
+#      src/Declare.hpp:383); Init2d is not called by the harness.
+# No header, library or tool is substituted: Eigen is the reference's own
+# vendored copy (include/Eigen), compiled as is.
+set -euo pipefail
+REF=${VVM_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)
+OUT=$(cd "$HERE/.." && pwd)/_ref
+if [ ! -d "$REF/src" ]; then
+    echo "build_ref: no reference sources at $REF; skipping" >&2
+    exit 0
+fi
+mkdir -p "$OUT"
+STAGE="$OUT/stage.$$"
+rm -rf "$STAGE"
+mkdir -p "$STAGE"
+trap 'rm -rf "$STAGE"' EXIT
+
+FILES="Config.hpp Declare.hpp Timer.hpp Iteration.cpp Advection.cpp Buoyancy.cpp Boundary.cpp \
+PoissonSolver.cpp Turbulent.cpp MicroPhysics.cpp NumericalProcess.cpp AddForcing.cpp Init.cpp"
+for f in $FILES; do cp "$REF/src/$f" "$STAGE/$f"; done
+
+cfg="$STAGE/Config.hpp"
+for flag in OUTPUTNC OUTPUTNCSAMEFILE P3_MICROPHY RTERRTMGP LOADFILE; do
+    grep -q "^#define $flag\b" "$cfg" || { echo "build_ref: Config.hpp has no '#define $flag'" >&2; exit 1; }
+    sed -i "s|^#define $flag\b|// #define $flag|" "$cfg"
+done
+grep -q '^// #define KESSLER_MICROPHY' "$cfg" || { echo "build_ref: KESSLER_MICROPHY line not found" >&2; exit 1; }
+sed -i 's|^// #define KESSLER_MICROPHY|#define KESSLER_MICROPHY|' "$cfg"
+
+af="$STAGE/AddForcing.cpp"
+[ "$(grep -c 'radiation_heating_rate' "$af")" = 1 ] || { echo "build_ref: AddForcing.cpp changed" >&2; exit 1; }
+sed -i '/radiation_heating_rate/d' "$af"
+
+ini="$STAGE/Init.cpp"
+ln=$(grep -n 'model.pb_lev_all\[i\]\[k\] = model.pb_lev\[k\];' "$ini" | cut -d: -f1)
+[ -n "$ln" ] || { echo "build_ref: Init.cpp pb_lev_all loop not found" >&2; exit 1; }
+# the loop is 'for i {' / 'for k {' / assignment / '}' / '}'
+sed -i "$((ln - 2)),$((ln + 2))d" "$ini"
+if grep -q pb_lev_all "$ini"; then echo "build_ref: Init.cpp edit failed" >&2; exit 1; fi
+
+CXX=${CXX:-g++}
+SRCS=""
+for f in $FILES; do case $f in *.cpp) SRCS="$SRCS $STAGE/$f";; esac; done
+# -fno-fast-math / -ffp-contract=off: IEEE evaluation in source order.
+$CXX -O3 -std=c++17 -ffp-contract=off -w -I"$REF/include" -I"$STAGE" \
+    $SRCS "$HERE/ref_harness.cpp" -o "$OUT/vvm_ref.tmp"
+mv "$OUT/vvm_ref.tmp" "$OUT/vvm_ref"
+echo "build_ref: built $OUT/vvm_ref"
