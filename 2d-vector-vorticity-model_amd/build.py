@@ -17,7 +17,8 @@ OUT = os.path.join(PKG, "libvvm_amd.so")
 OBJ = os.path.join(PKG, "_build")
 
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"-I{ROOT}/include", f"-I{PKG}/csrc", "-Wall",
-          "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
+          "-Wno-unused-function", "-Wno-unused-variable", "-Wno-unused-but-set-variable", "-Wno-unused-value",
+          "-Wno-unused-result"]
 # Device code: gfx950 only; packed-FP32 instructions off; the FP64 model
 # kernels keep the reference's rounding (no FMA contraction), set per file
 # with #pragma clang fp contract where needed.

@@ -1,0 +1,358 @@
+// k_dynamics.hip -- advection, buoyancy, rain sedimentation, the negative
+// value fixer, forcing and halos of one VVM timestep (phase A of
+// vvm::Iteration::TimeMarching, src/Iteration.cpp:263-285), plus
+// PoissonSolver::pubarTop_pt.
+//
+// FP64 throughout; no FMA contraction, so every expression rounds as the
+// reference's does (the oracle and the compiled reference agree bit for bit
+// on these operators; the GPU differs only where the device math library's
+// pow/sqrt/exp do).
+#pragma clang fp contract(off)
+
+#include "vvm_internal.h"
+
+namespace vvm {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__device__ inline double plus_(double v) { return 0.5 * (v + fabs(v)); }
+__device__ inline double minu_(double v) { return 0.5 * (v - fabs(v)); }
+
+// Third-order upwind-biased correction of one face flux, Advection.cpp:40-50
+// (rho0/v0 at the face, rhom/vm one face upstream, rhop/vp one downstream;
+// fm2..fp1 the four cell values around the face).
+__device__ inline double upwind3(double rho0, double v0, double rhom, double vm, double rhop, double vp, double fm2,
+                                 double fm1, double f0, double fp1) {
+    const double p0 = plus_(v0), pm = plus_(vm), m0 = minu_(v0), mp = minu_(vp);
+    return -1. / 3. *
+           (rho0 * p0 * (f0 - fm1) - sqrt(rho0 * p0 * rhom * pm) * (fm1 - fm2) - rho0 * m0 * (f0 - fm1) -
+            sqrt(fabs(rho0 * m0 * rhop * mp)) * (fp1 - f0));
+}
+
+__device__ inline bool corr_window(const Grid &G, int i, int k) {
+    return i >= 2 && i <= G.nx - 3 && k >= 2 && k <= G.nz - 3;
+}
+
+// Scalar x-face flux at (i,k) (between i-1 and i), Advection.cpp:36,40-44.
+__device__ inline double sflux_x(const Grid &G, const double *__restrict__ f, const double *__restrict__ u,
+                                 const double *__restrict__ rhou, int i, int k) {
+    const double r = rhou[k], ui = u[ix(G, i, k)];
+    const double f0 = f[ix(G, i, k)], fm1 = f[ix(G, i - 1, k)];
+    double fl = r * ui * (f0 + fm1);
+    if (corr_window(G, i, k))
+        fl += upwind3(r, ui, r, u[ix(G, i - 1, k)], r, u[ix(G, i + 1, k)], f[ix(G, i - 2, k)], fm1, f0,
+                      f[ix(G, i + 1, k)]);
+    return fl;
+}
+
+// Scalar z-face flux at (i,k) (between k-1 and k), Advection.cpp:37,46-50.
+__device__ inline double sflux_z(const Grid &G, const double *__restrict__ f, const double *__restrict__ w,
+                                 const double *__restrict__ rhow, int i, int k) {
+    const double r = rhow[k], wk = w[ix(G, i, k)];
+    const double f0 = f[ix(G, i, k)], fm1 = f[ix(G, i, k - 1)];
+    double fl = r * wk * (f0 + fm1);
+    if (corr_window(G, i, k))
+        fl += upwind3(r, wk, rhow[k - 1], w[ix(G, i, k - 1)], rhow[k + 1], w[ix(G, i, k + 1)], f[ix(G, i, k - 2)],
+                      fm1, f0, f[ix(G, i, k + 1)]);
+    return fl;
+}
+
+// Corner velocities of Advection_zeta, Advection.cpp:90-91.
+__device__ inline double Uw(const Grid &G, const double *u, const double *rhou, int i, int k) {
+    return 0.25 * (rhou[k] * (u[ix(G, i + 1, k)] + u[ix(G, i, k)]) +
+                   rhou[k - 1] * (u[ix(G, i + 1, k - 1)] + u[ix(G, i, k - 1)]));
+}
+__device__ inline double Wu(const Grid &G, const double *w, const double *rhow, int i, int k) {
+    return 0.25 * (rhow[k + 1] * (w[ix(G, i, k + 1)] + w[ix(G, i - 1, k + 1)]) +
+                   rhow[k] * (w[ix(G, i, k)] + w[ix(G, i - 1, k)]));
+}
+
+// Vorticity fluxes, Advection.cpp:107-118 (the density already sits in U_w/W_u).
+__device__ inline double zflux_x(const Grid &G, const double *z, const double *u, const double *rhou, int i, int k) {
+    const double U = Uw(G, u, rhou, i, k);
+    const double z0 = z[ix(G, i, k)], z1 = z[ix(G, i + 1, k)];
+    double fl = U * (z1 + z0);
+    if (corr_window(G, i, k)) {
+        const double Pu = plus_(U), Pum = plus_(Uw(G, u, rhou, i - 1, k));
+        const double Mu = minu_(U), Mup = minu_(Uw(G, u, rhou, i + 1, k));
+        fl += -1. / 3. *
+              (Pu * (z1 - z0) - sqrt(Pu * Pum) * (z0 - z[ix(G, i - 1, k)]) - Mu * (z1 - z0) -
+               sqrt(fabs(Mu * Mup)) * (z[ix(G, i + 2, k)] - z1));
+    }
+    return fl;
+}
+__device__ inline double zflux_z(const Grid &G, const double *z, const double *w, const double *rhow, int i, int k) {
+    const double W = Wu(G, w, rhow, i, k);
+    const double z0 = z[ix(G, i, k)], z1 = z[ix(G, i, k + 1)];
+    double fl = W * (z1 + z0);
+    if (corr_window(G, i, k)) {
+        const double Pw = plus_(W), Pwm = plus_(Wu(G, w, rhow, i, k - 1));
+        const double Mw = minu_(W), Mwp = minu_(Wu(G, w, rhow, i, k + 1));
+        fl += -1. / 3. *
+              (Pw * (z1 - z0) - sqrt(Pw * Pwm) * (z0 - z[ix(G, i, k - 1)]) - Mw * (z1 - z0) -
+               sqrt(fabs(Mw * Mwp)) * (z[ix(G, i, k + 2)] - z1));
+    }
+    return fl;
+}
+
+__device__ inline double thv(const Grid &G, const double *th, const double *qv, int i, int k) {
+    return th[ix(G, i, k)] * (1 + 0.608 * qv[ix(G, i, k)]);
+}
+
+// AB2 update of one history slot: writes d to slot sn (and to slot 0 on the
+// reference's step-0 start, Advection.cpp:70) and returns the old slot.
+__device__ inline double ab2_hist(double *const (&h)[2], size_t p, int step, double d) {
+    const int sn = (step + 1) % 2, so = step % 2;
+    h[sn][p] = d;
+    if (step == 0) h[0][p] = d;
+    return h[so][p];
+}
+
+// ------------------------------------------------------------------ zeta
+
+// Iteration::pzeta_pt: Advection_zeta (Advection.cpp:84-149) + Bouyancy
+// (Buoyancy.cpp:11-58) + BoundaryProcess2D_westdown(zetap).
+__global__ void __launch_bounds__(BX *BY) k_pzeta(Grid G, StepPtrs S, int step) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    double *zp = S.fut[F_ZETA];
+    if (k == 0 || k == 1 || k == G.nz - 1) {
+        zp[ix(G, i, k)] = 0.;
+        return;
+    }
+    if (i < 1 || i > G.nx - 2) return;
+    const double *z = S.now[F_ZETA], *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
+    const size_t p = ix(G, i, k);
+    // flux_u at i-1 = 0 is the periodic image of nx-2 (BoundaryProcess2D_center on flux_u)
+    const double fu0 = zflux_x(G, z, u, rhou, i, k);
+    const double fum = zflux_x(G, z, u, rhou, (i == 1) ? G.nx - 2 : i - 1, k);
+    const double fw0 = zflux_z(G, z, w, rhow, i, k);
+    const double fwm = zflux_z(G, z, w, rhow, i, k - 1);
+    const double px = (fu0 - fum) * G.r2dx / rhow[k];
+    const double pz = (fw0 - fwm) * G.r2dz / rhow[k];
+    const double dadv = -px - pz;
+    const double dold = ab2_hist(S.hist[H_ZETA_ADV], p, step, dadv);
+    double zeta_new = z[p] + 1.5 * G.dt * dadv - 0.5 * G.dt * dold;
+
+    const double *th = S.now[F_TH], *qv = S.now[F_QV], *qc = S.now[F_QC], *qr = S.now[F_QR], *thvb = S.P.thvb;
+    const double gth = G.GRAVITY / rhow[k] * 0.5 *
+                       ((thv(G, th, qv, i, k) - thv(G, th, qv, i - 1, k)) / thvb[k] +
+                        (thv(G, th, qv, i, k - 1) - thv(G, th, qv, i - 1, k - 1)) / thvb[k - 1]) *
+                       G.rdx;
+    const double gqc = G.GRAVITY / rhow[k] *
+                       (0.5 * (qc[ix(G, i, k)] + qc[ix(G, i, k - 1)]) - 0.5 * (qc[ix(G, i - 1, k)] + qc[ix(G, i - 1, k - 1)])) *
+                       G.rdx;
+    const double gqr = G.GRAVITY / rhow[k] *
+                       (0.5 * (qr[ix(G, i, k)] + qr[ix(G, i, k - 1)]) - 0.5 * (qr[ix(G, i - 1, k)] + qr[ix(G, i - 1, k - 1)])) *
+                       G.rdx;
+    double db = gth;
+    db += -gqc - gqr;
+    const double bold = ab2_hist(S.hist[H_TH_BUOY], p, step, db);
+    zeta_new += (1.5 * G.dt * db) - (0.5 * G.dt * bold);
+    store_westdown(zp, G, i, k, zeta_new);
+}
+
+// ----------------------------------------------------------- scalars
+
+// Advection_thermo (Advection.cpp:25-81) for one field at interior (i,k).
+__device__ inline double thermo_tend(const Grid &G, const double *f, const double *u, const double *w,
+                                     const double *rhou, const double *rhow, int i, int k) {
+    const double fu1 = sflux_x(G, f, u, rhou, (i + 1 == G.nx - 1) ? 1 : i + 1, k);
+    const double fu0 = sflux_x(G, f, u, rhou, i, k);
+    const double fw1 = (k + 1 == G.nz - 1) ? 0. : sflux_z(G, f, w, rhow, i, k + 1);
+    const double fw0 = sflux_z(G, f, w, rhow, i, k);
+    const double px = (fu1 - fu0) * G.r2dx / rhou[k];
+    const double pz = (fw1 - fw0) * G.r2dz / rhou[k];
+    return -px - pz;
+}
+
+// Rain sedimentation flux of Advection_qrVT (Advection.cpp:165-166) at (i,k).
+__device__ inline double vt_flux(const Grid &G, const double *qr, const double *rhow, const double *vtfac, int i,
+                                 int k) {
+    const double s = qr[ix(G, i, k)] + qr[ix(G, i, k - 1)];
+    const double VT = 1E-2 * (3634 * pow(1E-3 * rhow[k] * 0.5 * s, 0.1346) * vtfac[k]);
+    return rhow[k] * VT * s;
+}
+
+// pth_pt, pqv_pt, pqc_pt, pqr_pt (src/Iteration.cpp:36-40, 90-124): the four
+// scalar advections sharing u and w, rain sedimentation on qr, surface
+// precipitation, and the first pass of NegativeValueProcess (zero the
+// negatives, sum positives and negatives) for qv, qc, qr.  The last block
+// turns the sums into rescale ratios.
+__global__ void __launch_bounds__(BX *BY) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
+    double part[6] = {0., 0., 0., 0., 0., 0.};
+    if (inside) {
+        const double *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
+        const size_t p = ix(G, i, k);
+        // theta
+        {
+            const double d = thermo_tend(G, S.now[F_TH], u, w, rhou, rhow, i, k);
+            const double dold = ab2_hist(S.hist[H_TH_ADV], p, step, d);
+            S.fut[F_TH][p] = S.now[F_TH][p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
+        }
+        const int hid[3] = {H_QV_ADV, H_QC_ADV, H_QR_ADV};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int f = F_QV + q;
+            const double d = thermo_tend(G, S.now[f], u, w, rhou, rhow, i, k);
+            const double dold = ab2_hist(S.hist[hid[q]], p, step, d);
+            double v = S.now[f][p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
+            if (f == F_QR) {
+                const double *qr = S.now[F_QR];
+                const double f1 = (k + 1 == G.nz - 1) ? 0. : vt_flux(G, qr, rhow, S.P.vtfac, i, k + 1);
+                const double f0 = vt_flux(G, qr, rhow, S.P.vtfac, i, k);
+                const double dvt = (f1 - f0) * G.r2dz / rhou[k];
+                const double vold = ab2_hist(S.hist[H_QR_VT], p, step, dvt);
+                v += 1.5 * G.dt * dvt - 0.5 * G.dt * vold;
+            }
+            if (v >= 0.) part[2 * q] += v;
+            else {
+                part[2 * q + 1] += v;
+                v = 0.;
+            }
+            S.fut[f][p] = v;
+        }
+        if (k == 1) {
+            // surface precipitation, src/Iteration.cpp:114-119 (now-qr, last w)
+            const double r1 = rhou[1];
+            const double q1 = S.now[F_QR][ix(G, i, 1)];
+            const double VT = 1E-2 * (3634 * pow(1E-3 * r1 * q1, 0.1346) * S.s[S_PRECIP_FAC]);
+            const double rain = -r1 * (0.5 * (w[ix(G, i, 2)] + 0.) - VT) * q1;
+            S.precip[i] = (rain < 0) ? 0. : rain;
+        }
+    }
+    double tot[6];
+    if (grid_sum_last<6>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            for (int q = 0; q < 3; q++) {
+                const double P = tot[2 * q], N = tot[2 * q + 1];
+                const bool skip = (P == 0. || fabs(N) > P);
+                S.s[S_NEG_RATIO + q] = skip ? -1. : 1. - fabs(N / P);
+            }
+        }
+    }
+}
+
+// Second pass of NegativeValueProcess for qv, qc, qr (MicroPhysics.cpp:
+// 129-137), AddForcing (AddForcing.cpp:6-28: Q1LS/Q2LS and the random
+// theta kick) when `forcing`, and the center halos of thp, qvp, qcp, qrp.
+__global__ void __launch_bounds__(BX *BY) k_scale_forcing(Grid G, StepPtrs S, int forcing, int tropical,
+                                                          int random_kick) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const size_t p = ix(G, i, k);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const int f = F_QV + q;
+        double v = S.fut[f][p];
+        const double ratio = S.s[S_NEG_RATIO + q];
+        if (ratio >= 0. && v > 0) v = v * ratio;
+        if (forcing && tropical && f == F_QV) v += G.dt * S.P.Q2LS[k];
+        store_center(S.fut[f], G, i, k, v);
+    }
+    double th = S.fut[F_TH][p];
+    if (forcing) {
+        if (tropical) th += G.dt * S.P.Q1LS[k];
+        if (random_kick) th += G.dt * S.forcing[p];
+    }
+    store_center(S.fut[F_TH], G, i, k, th);
+}
+
+// Halo value of BoundaryProcess2D_center at a perimeter cell.
+__device__ inline size_t center_src(const Grid &G, int i, int k) {
+    const int is = (i == 0) ? G.nx - 2 : (i == G.nx - 1) ? 1 : i;
+    const int ks = (k == 0) ? 1 : (k == G.nz - 1) ? G.nz - 2 : k;
+    return ix(G, is, ks);
+}
+
+// BoundaryProcess2D_center / _westdown on the perimeter of a field.
+__device__ inline void bc_center_cell(double *a, const Grid &G, int i, int k) {
+    if (i == 0 || i == G.nx - 1 || k == 0 || k == G.nz - 1) a[ix(G, i, k)] = a[center_src(G, i, k)];
+}
+__device__ inline void bc_westdown_cell(double *a, const Grid &G, int i, int k) {
+    if (k == 0 || k == 1 || k == G.nz - 1) a[ix(G, i, k)] = 0.;
+    else if (i == 0) a[ix(G, i, k)] = a[ix(G, G.nx - 2, k)];
+    else if (i == G.nx - 1) a[ix(G, i, k)] = a[ix(G, 1, k)];
+}
+
+__global__ void __launch_bounds__(BX *BY) k_bc_uw(Grid G, StepPtrs S) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    bc_center_cell(S.u, G, i, k);
+    bc_westdown_cell(S.w, G, i, k);
+}
+
+// BoundaryProcess2D_all (src/Boundary.cpp:33-52), Kessler build.
+__global__ void __launch_bounds__(BX *BY) k_bc_all(Grid G, StepPtrs S) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    bc_westdown_cell(S.fut[F_ZETA], G, i, k);
+    bc_center_cell(S.fut[F_TH], G, i, k);
+    bc_westdown_cell(S.w, G, i, k);
+    bc_center_cell(S.u, G, i, k);
+    bc_center_cell(S.fut[F_QV], G, i, k);
+    bc_center_cell(S.fut[F_QC], G, i, k);
+    bc_center_cell(S.fut[F_QR], G, i, k);
+}
+
+// PoissonSolver::pubarTop_pt (src/PoissonSolver.cpp:655-676): domain-mean
+// vertical momentum flux and K_M shear at the top, AB2 for ubarTopp.
+__global__ void __launch_bounds__(1024) k_pubartop(Grid G, StepPtrs S, int step) {
+    const int nx = G.nx, nz = G.nz;
+    const double *u = S.u, *w = S.w, *K = S.RKM, *rhou = S.P.rhou, *rhow = S.P.rhow;
+    double acc[3] = {0., 0., 0.};
+    for (int i = 1 + threadIdx.x; i < nx - 1; i += blockDim.x) {
+        acc[0] += 0.25 * (rhou[nz - 2] * u[ix(G, i, nz - 2)] + rhou[nz - 3] * u[ix(G, i, nz - 3)]) *
+                  (w[ix(G, i, nz - 2)] + w[ix(G, i - 1, nz - 2)]);
+        acc[1] += rhow[nz - 1] * 0.25 *
+                  (K[ix(G, i, nz - 1)] + K[ix(G, i, nz - 2)] + K[ix(G, i - 1, nz - 1)] + K[ix(G, i - 1, nz - 2)]) *
+                  (G.rdz * (u[ix(G, i, nz - 1)] - u[ix(G, i, nz - 2)]));
+        acc[2] += rhow[nz - 2] * 0.25 *
+                  (K[ix(G, i, nz - 2)] + K[ix(G, i, nz - 3)] + K[ix(G, i - 1, nz - 2)] + K[ix(G, i - 1, nz - 3)]) *
+                  (G.rdx * (w[ix(G, i, nz - 2)] - w[ix(G, i - 1, nz - 2)]) +
+                   G.rdz * (u[ix(G, i, nz - 2)] - u[ix(G, i, nz - 3)]));
+    }
+    block_sum<3>(acc);
+    if (threadIdx.x == 0) {
+        const double n = (double)(nx - 2.);
+        const double down = acc[0] / n, kup = acc[1] / n, kdown = acc[2] / n;
+        const double p1 = (-down) * G.rdz / rhou[nz - 2];
+        const double p2 = (kup - kdown) * G.rdz / rhou[nz - 2];
+        const int sn = (step + 1) % 2, so = step % 2;
+        double *dub = S.s + S_DUBAR0;
+        dub[sn] = -p1 + p2;
+        if (step == 0) dub[0] = dub[1];
+        S.s[S_UBARTOPP] = S.s[S_UBARTOP] + 1.5 * G.dt * dub[sn] - 0.5 * G.dt * dub[so];
+    }
+}
+
+inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY - 1) / BY); }
+
+}  // namespace
+
+void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
+    hipLaunchKernelGGL(k_pzeta, grid2d(G), dim3(BX, BY), 0, st, G, S, step);
+}
+void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st) {
+    hipLaunchKernelGGL(k_thermo4, grid2d(G), dim3(BX, BY), 0, st, G, S, step, rs);
+}
+void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_scale_forcing, grid2d(G), dim3(BX, BY), 0, st, G, S, int(forcing), int(tropical),
+                       int(random_kick));
+}
+void launch_bc_uw(const Grid &G, const StepPtrs &S, hipStream_t st) {
+    hipLaunchKernelGGL(k_bc_uw, grid2d(G), dim3(BX, BY), 0, st, G, S);
+}
+void launch_bc_all_prog(const Grid &G, const StepPtrs &S, hipStream_t st) {
+    hipLaunchKernelGGL(k_bc_all, grid2d(G), dim3(BX, BY), 0, st, G, S);
+}
+void launch_pubartop(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
+    hipLaunchKernelGGL(k_pubartop, dim3(1), dim3(1024), 0, st, G, S, step);
+}
+
+}  // namespace vvm

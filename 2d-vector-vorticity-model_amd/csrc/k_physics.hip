@@ -1,0 +1,338 @@
+// k_physics.hip -- Kessler microphysics, first-order turbulence closure and
+// K-diffusion, nudging and horizontal means of one VVM timestep (phases C-E
+// of vvm::Iteration::TimeMarching, src/Iteration.cpp:300-434).
+//
+// FP64, no FMA contraction (reference rounding, see k_dynamics.hip).
+// std::max(a,b) / std::min(a,b) keep their NaN behaviour: (a<b)?b:a and
+// (b<a)?b:a.
+#pragma clang fp contract(off)
+
+#include "vvm_internal.h"
+
+namespace vvm {
+
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+__device__ inline double maxs(double a, double b) { return (a < b) ? b : a; }
+__device__ inline double mins(double a, double b) { return (b < a) ? b : a; }
+
+inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY - 1) / BY); }
+
+// MicroPhysics::autoconversion, accretion, evaporation, condensation
+// (MicroPhysics.cpp:29-51, 54-77, 80-114, 7-26).  Each is pointwise, so
+// running the four in call order per point equals the reference's four
+// sweeps.  Then the first pass of NegativeValueProcess on qvp, qcp, qrp
+// (src/Iteration.cpp:309-311); the last block computes the three ratios.
+__global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot rs) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
+    double part[6] = {0., 0., 0., 0., 0., 0.};
+    if (inside) {
+        const size_t p = ix(G, i, k);
+        const double dt = G.dt, pib = S.P.pib[k], rhou = S.P.rhou[k], pb = S.P.pb[k], pc = S.P.pcoef[k];
+        double qc = S.fut[F_QC][p], qr = S.fut[F_QR][p], qv = S.fut[F_QV][p], th = S.fut[F_TH][p];
+        {  // autoconversion
+            const double qcplus = maxs(0., qc);
+            double ar = 0.001 * (qcplus - 0.001);
+            ar = maxs(0., ar);
+            const double arcrdt = mins(ar * dt, qcplus);
+            qc = qc - arcrdt;
+            qr = qr + arcrdt;
+            S.autoc[p] = arcrdt;
+        }
+        {  // accretion
+            const double qcplus = maxs(0., qc), qrplus = maxs(0., qr);
+            const double cr = 2.2 * qcplus * (pow(qrplus, 0.875));
+            const double arcrdt = mins(cr * dt, qcplus);
+            qc = qc - arcrdt;
+            qr = qr + arcrdt;
+            S.accr[p] = arcrdt;
+        }
+        {  // rain evaporation
+            const double qrplus = maxs(0., qr), qvplus = maxs(0., qv);
+            const double qvs = pc * exp(17.27 * (pib * th - 273.) / (pib * th - 36.));
+            const double deficit = maxs((1. - qvplus / qvs), 0.);
+            const double coef = 1.6 + 124.9 * pow((1E-3 * rhou * qrplus), 0.2046);
+            const double er = coef * deficit * (pow(1E-3 * rhou * qrplus, 0.525)) /
+                              ((5.4E5 + 2.55E6 / (1E-2 * pb * qvs)) * 1E-3 * rhou);
+            const double erdt = mins(qrplus, maxs(0., er * dt));
+            qr = qr - erdt;
+            qv = qv + erdt;
+            th = th - G.Lv * erdt / (G.Cp * pib);
+            S.evap[p] = erdt;
+        }
+        {  // saturation adjustment
+            const double qvs = pc * exp(17.27 * (pib * th - 273.) / (pib * th - 36.));
+            const double tm = th * pib - 36.;
+            const double phi = qvs * (17.27 * 237. * G.Lv) / (G.Cp * (tm * tm));
+            double C = (qv - qvs) / (1 + phi);
+            if (fabs(C) > qc && C < 0) C = -qc;
+            qv = qv - C;
+            qc = qc + C;
+            th = th + G.Lv / (G.Cp * pib) * C;
+            S.cond[p] = C;
+        }
+        double q3[3] = {qv, qc, qr};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (q3[q] >= 0.) part[2 * q] += q3[q];
+            else {
+                part[2 * q + 1] += q3[q];
+                q3[q] = 0.;
+            }
+        }
+        S.fut[F_QV][p] = q3[0];
+        S.fut[F_QC][p] = q3[1];
+        S.fut[F_QR][p] = q3[2];
+        S.fut[F_TH][p] = th;
+    }
+    double tot[6];
+    if (grid_sum_last<6>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            for (int q = 0; q < 3; q++) {
+                const double P = tot[2 * q], N = tot[2 * q + 1];
+                S.s[S_NEG_RATIO + q] = (P == 0. || fabs(N) > P) ? -1. : 1. - fabs(N / P);
+            }
+        }
+    }
+}
+
+// Turbulence::RKM_RKH coefficient part (Turbulent.cpp:13-45), with the
+// center halos of RKM and RKH.
+__global__ void __launch_bounds__(BX *BY) k_turb_coeffs(Grid G, StepPtrs S) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const double *u = S.u, *w = S.w, *thp = S.fut[F_TH];
+    const double Rz =
+        0.5 * ((w[ix(G, i + 1, k + 1)] + w[ix(G, i + 1, k)]) - (w[ix(G, i - 1, k + 1)] + w[ix(G, i - 1, k)])) * G.r2dx +
+        0.5 * ((u[ix(G, i + 1, k + 1)] + u[ix(G, i, k + 1)]) - (u[ix(G, i + 1, k - 1)] + u[ix(G, i, k - 1)])) * G.r2dz;
+    const double dv = (u[ix(G, i + 1, k)] - u[ix(G, i, k)]) * G.rdx + (w[ix(G, i, k + 1)] - w[ix(G, i, k)]) * G.rdz;
+    const double Ro = dv * dv;
+    const double Ri = (G.GRAVITY / thp[ix(G, i, k)] * (thp[ix(G, i, k + 1)] - thp[ix(G, i, k - 1)]) * G.r2dz) /
+                      (Rz * Rz + 2. * Ro);
+    const double l2 = S.P.lambda2[k];
+    double km, kh;
+    if (Ri < 0) {
+        km = l2 * sqrt(Rz * Rz + 2. * Ro) * sqrt(1. - 16. * Ri);
+        kh = l2 * sqrt(Rz * Rz + 2. * Ro) * 1.4 * sqrt(1. - 40. * Ri);
+    } else if (0 < Ri && Ri < 0.25) {
+        km = l2 * sqrt(Rz * Rz + 2. * Ro) * pow(1. - 4. * Ri, 4.);
+        kh = l2 * sqrt(Rz * Rz + 2. * Ro) * 1.4 * (1. - 1.2 * Ri) * pow(1. - 4. * Ri, 4.);
+    } else {
+        km = 0.;
+        kh = 0.;
+    }
+    const double kmax = 0.8 * G.dx * G.dz / G.dt;
+    km = maxs(km, 10.);
+    kh = maxs(kh, 10.);
+    km = mins(km, kmax);
+    kh = mins(kh, kmax);
+    store_center(S.RKM, G, i, k, km);
+    store_center(S.RKH, G, i, k, kh);
+}
+
+// ------------------------------------------------------------ diffusion
+//
+// Turbulence::Mparam / Hparam (Turbulent.cpp:68-114) update a field in
+// place, k outer and i inner, so each new value depends on the *new* west
+// and lower neighbours: new(i,k) = a(i,k) new(i-1,k) + e(i,k), where e holds
+// the old centre/east/upper values and the new lower value.  Ghost cells are
+// never updated during the sweep.  Row k is therefore a first-order linear
+// recurrence in i once row k-1 is final: one workgroup per field walks the
+// rows in order and solves each row with a parallel scan of affine maps
+// y -> a y + e.  The values agree with the sequential sweep to rounding.
+// The Rayleigh nudging of theta and zeta (NumericalProcess.cpp:60-84),
+// applied after the sweep in the reference, is fused into the store: the
+// recurrence keeps the un-nudged values.
+
+constexpr int DT = 1024;  // threads per field workgroup
+
+struct Affine {
+    double a, b;  // y = a x + b
+};
+
+// g after f: y = g.a (f.a x + f.b) + g.b
+__device__ inline Affine after(const Affine &g, const Affine &f) { return {g.a * f.a, g.a * f.b + g.b}; }
+
+// Exclusive scan over the block; returns the composition of the maps of all
+// lower threads (identity for thread 0).
+__device__ inline Affine block_exclusive_scan(Affine m) {
+    __shared__ double sa[DT / kWave], sb[DT / kWave];
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
+    Affine inc = m;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double pa = __shfl_up(inc.a, off, kWave), pb = __shfl_up(inc.b, off, kWave);
+        if (lane >= off) inc = after(inc, Affine{pa, pb});
+    }
+    if (lane == kWave - 1) {
+        sa[wid] = inc.a;
+        sb[wid] = inc.b;
+    }
+    __syncthreads();
+    Affine wpre{1., 0.};
+    for (int w2 = 0; w2 < wid; w2++) wpre = after(Affine{sa[w2], sb[w2]}, wpre);
+    const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
+    Affine ex = (lane == 0) ? wpre : after(Affine{ea, eb}, wpre);
+    __syncthreads();
+    return ex;
+}
+
+// E consecutive columns per thread (compile-time so the row state stays in
+// registers); E * DT >= nx - 2.
+template <int E>
+__global__ void __launch_bounds__(DT) k_diffusion(Grid G, StepPtrs S, int nudge) {
+    const int fld = blockIdx.x;  // 0: zeta (Mparam), 1..4: th, qv, qc, qr (Hparam)
+    const int fidx[5] = {F_ZETA, F_TH, F_QV, F_QC, F_QR};
+    double *f = S.fut[fidx[fld]];
+    const bool mpar = (fld == 0);
+    const double *K = mpar ? S.RKM : S.RKH;
+    const double *rhou = S.P.rhou, *rhow = S.P.rhow;
+    // nudge bit 0: Nudge_theta, bit 1: Nudge_zeta
+    const double *rate = (fld == 0) ? ((nudge & 2) ? S.P.rate_z : nullptr)
+                                    : ((fld == 1 && (nudge & 1)) ? S.P.rate_th : nullptr);
+    const double *thb_init = S.P.thb_init;
+    const int n = G.nx - 2;
+    const int i0 = 1 + threadIdx.x * E;
+    const double A = G.rdx2 * G.dt;
+
+    double ynew[E];  // this thread's new values of the previous row (pre-nudge)
+    for (int k = 1; k <= G.nz - 2; k++) {
+        const double Bk = mpar ? G.rdz2 * G.dt / rhow[k] : G.rdz2 * G.dt / rhou[k];
+        double a[E], e[E];
+        Affine m{1., 0.};
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int i = i0 + j;
+            a[j] = 0.;
+            e[j] = 0.;
+            if (i > n) continue;
+            const double old = f[ix(G, i, k)], oldE = f[ix(G, i + 1, k)], oldT = f[ix(G, i, k + 1)];
+            const double yB = (k == 1) ? f[ix(G, i, 0)] : ynew[j];
+            double Ke, Kw, Ct, Cb;
+            if (mpar) {
+                Ke = 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
+                Kw = 0.5 * (K[ix(G, i - 1, k)] + K[ix(G, i - 1, k - 1)]);
+                Ct = rhou[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
+                Cb = rhou[k - 1] * 0.5 * (K[ix(G, i, k - 1)] + K[ix(G, i - 1, k - 1)]);
+            } else {
+                Ke = 0.5 * (K[ix(G, i + 1, k)] + K[ix(G, i, k)]);
+                Kw = 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
+                Ct = rhow[k + 1] * 0.5 * (K[ix(G, i, k + 1)] + K[ix(G, i, k)]);
+                Cb = rhow[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
+            }
+            a[j] = A * Kw;
+            e[j] = old + A * (Ke * (oldE - old) - Kw * old) + Bk * (Ct * (oldT - old) - Cb * (old - yB));
+            m = after(Affine{a[j], e[j]}, m);
+        }
+        const double y0 = f[ix(G, 0, k)];  // ghost: old value, as in the reference
+        const Affine pre = block_exclusive_scan(m);
+        double y = pre.a * y0 + pre.b;
+        const double r = rate ? rate[k] : 0.;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int i = i0 + j;
+            if (i > n) continue;
+            y = a[j] * y + e[j];
+            ynew[j] = y;
+            double out = y;
+            if (r != 0.) out = (fld == 1) ? y - r * (y - thb_init[k]) : y - r * y;
+            f[ix(G, i, k)] = out;
+        }
+        __syncthreads();
+    }
+}
+
+// NumericalProcess::Nudge_qv (NumericalProcess.cpp:86-95), all points.
+__global__ void __launch_bounds__(BX *BY) k_nudge_qv(Grid G, StepPtrs S, double tau) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    const size_t p = ix(G, i, k);
+    S.fut[F_QV][p] = S.fut[F_QV][p] + G.dt * (S.P.qvb0[k] - S.now[F_QV][p]) / tau;
+}
+
+// Iteration::updateMean (src/Iteration.cpp:42-86) as called by
+// nextTimeStep: one block per level.  thb_zeta is not read by the step and
+// is formed on download.
+__global__ void __launch_bounds__(256) k_update_mean(Grid G, StepPtrs S, int shift_ubar) {
+    const int k = blockIdx.x + 1;
+    const double *thp = S.fut[F_TH], *qvp = S.fut[F_QV];
+    double acc[2] = {0., 0.};
+    for (int i = 1 + threadIdx.x; i < G.nx - 1; i += blockDim.x) {
+        acc[0] += thp[ix(G, i, k)];
+        acc[1] += qvp[ix(G, i, k)];
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        double *thb = S.P.thb, *qvb = S.P.qvb, *thvb = S.P.thvb;
+        const double tb = acc[0] / (double)(G.nx - 2.), qb = acc[1] / (double)(G.nx - 2.);
+        const double tv = tb * (1 + 0.608 * qb);
+        S.P.thbm[k] = thb[k];
+        S.P.thvbm[k] = thvb[k];
+        thb[k] = tb;
+        qvb[k] = qb;
+        thvb[k] = tv;
+        if (k == 1) {
+            S.P.thbm[0] = thb[0];
+            S.P.thvbm[0] = thvb[0];
+            thb[0] = tb;
+            qvb[0] = qb;
+            thvb[0] = tv;
+            if (shift_ubar) {
+                // nextTimeStep's copy loop (src/Iteration.cpp:166-167)
+                S.s[S_UBARTOPM] = S.s[S_UBARTOPP];
+                S.s[S_UBARTOP] = S.s[S_UBARTOPP];
+            }
+        }
+        if (k == G.nz - 2) {
+            S.P.thbm[k + 1] = thb[k + 1];
+            S.P.thvbm[k + 1] = thvb[k + 1];
+            thb[k + 1] = tb;
+            qvb[k + 1] = qb;
+            thvb[k + 1] = tv;
+        }
+    }
+}
+
+}  // namespace
+
+void launch_kessler(const Grid &G, const StepPtrs &S, const RedSlot &rs, hipStream_t st) {
+    hipLaunchKernelGGL(k_kessler, grid2d(G), dim3(BX, BY), 0, st, G, S, rs);
+}
+void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st) {
+    hipLaunchKernelGGL(k_turb_coeffs, grid2d(G), dim3(BX, BY), 0, st, G, S);
+}
+bool diffusion_supported(const Grid &G) { return G.nx - 2 <= 8 * DT; }
+
+void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+    const int n = G.nx - 2;
+    if (n <= DT) hipLaunchKernelGGL(k_diffusion<1>, dim3(5), dim3(DT), 0, st, G, S, nudge);
+    else if (n <= 2 * DT) hipLaunchKernelGGL(k_diffusion<2>, dim3(5), dim3(DT), 0, st, G, S, nudge);
+    else if (n <= 4 * DT) hipLaunchKernelGGL(k_diffusion<4>, dim3(5), dim3(DT), 0, st, G, S, nudge);
+    else hipLaunchKernelGGL(k_diffusion<8>, dim3(5), dim3(DT), 0, st, G, S, nudge);
+}
+
+// Nudge_theta and Nudge_zeta alone (NumericalProcess.cpp:60-84), pointwise;
+// the timestep fuses them into the diffusion sweep instead.
+__global__ void __launch_bounds__(BX *BY) k_nudge(Grid G, StepPtrs S, int nudge) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const size_t p = ix(G, i, k);
+    const double rt = S.P.rate_th[k], rz = S.P.rate_z[k];
+    if ((nudge & 1) && rt != 0.) S.fut[F_TH][p] -= rt * (S.fut[F_TH][p] - S.P.thb_init[k]);
+    if ((nudge & 2) && rz != 0.) S.fut[F_ZETA][p] -= rz * (S.fut[F_ZETA][p]);
+}
+
+void launch_nudge(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+    hipLaunchKernelGGL(k_nudge, grid2d(G), dim3(BX, BY), 0, st, G, S, nudge);
+}
+void launch_nudge_qv(const Grid &G, const StepPtrs &S, double tau, hipStream_t st) {
+    hipLaunchKernelGGL(k_nudge_qv, grid2d(G), dim3(BX, BY), 0, st, G, S, tau);
+}
+void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipStream_t st) {
+    hipLaunchKernelGGL(k_update_mean, dim3(G.nz - 2), dim3(256), 0, st, G, S, int(shift_ubar));
+}
+
+}  // namespace vvm

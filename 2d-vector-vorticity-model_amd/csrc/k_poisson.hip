@@ -1,0 +1,834 @@
+// k_poisson.hip -- the elliptic part of a VVM timestep on the device:
+// PoissonSolver::cal_w (src/PoissonSolver.cpp:439-477) as multigrid-
+// preconditioned CG in FP64, or FP32 MG-PCG inside FP64 iterative
+// refinement, and PoissonSolver::cal_u (src/PoissonSolver.cpp:580-653).
+//
+// Multigrid: geometric, vertex coarsening (N -> N/2 periodic columns,
+// M -> (M-1)/2 Dirichlet rows), operators rediscretised with harmonic-mean
+// vertical conductances, bilinear prolongation P and restriction P^T,
+// damped-Jacobi V(2,2) smoothing and a single-workgroup coarsest solve.
+// Every piece is symmetric, so the V-cycle is a valid CG preconditioner.
+#pragma clang fp contract(off)
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "poisson.h"
+
+namespace vvm {
+
+namespace {
+
+constexpr int LX = 64, LY = 4;
+
+inline dim3 lgrid(int N, int M) { return dim3((N + LX - 1) / LX, (M + LY - 1) / LY); }
+
+template <typename T>
+__device__ inline T applyL(const T *__restrict__ x, int N, int i, int j, T cxj, T cuj, T cujm) {
+    const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
+    const size_t r = size_t(j) * N;
+    const T c = x[r + i];
+    return cxj * (T(2) * c - x[r + im] - x[r + ip]) + cuj * (c - x[r + N + i]) + cujm * (c - x[r - N + i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_jacobi_first(int N, int M, const T *b, T *x, const T *dinv, T omega) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    if (i >= N || j > M) return;
+    const size_t c = size_t(j) * N + i;
+    x[c] = omega * dinv[j] * b[c];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_jacobi(int N, int M, const T *b, const T *xin, T *xout, const T *cx,
+                                                  const T *cu, const T *dinv, T omega) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    if (i >= N || j > M) return;
+    const size_t c = size_t(j) * N + i;
+    xout[c] = xin[c] + omega * dinv[j] * (b[c] - applyL(xin, N, i, j, cx[j], cu[j], cu[j - 1]));
+}
+
+// Last post-smoothing sweep on level 0 fused with the CG dot product r.z.
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_jacobi_dot(int N, int M, const T *b, const T *xin, T *xout, const T *cx,
+                                                      const T *cu, const T *dinv, T omega, double *sc, RedSlot rs,
+                                                      int init) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (i < N && j <= M) {
+        const size_t c = size_t(j) * N + i;
+        const T z = xin[c] + omega * dinv[j] * (b[c] - applyL(xin, N, i, j, cx[j], cu[j], cu[j - 1]));
+        xout[c] = z;
+        part[0] = double(b[c]) * double(z);
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            if (!init) sc[P_BETA] = tot[0] / sc[P_RZ];
+            sc[P_RZ] = tot[0];
+        }
+    }
+}
+
+// Coarse right-hand side: b_c = P^T (b_f - L_f x_f).
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_resid_restrict(int Nf, int Mf, const T *bf, const T *xf, const T *cx,
+                                                          const T *cu, int Nc, int Mc, T *bc) {
+    const int ic = blockIdx.x * LX + threadIdx.x, jc = 1 + blockIdx.y * LY + threadIdx.y;
+    if (ic >= Nc || jc > Mc) return;
+    T s = 0;
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+        const int jf = 2 * jc + dj;
+        const T wj = (dj == 0) ? T(1) : T(0.5);
+#pragma unroll
+        for (int di = -1; di <= 1; di++) {
+            int i_f = 2 * ic + di;
+            if (i_f < 0) i_f += Nf;
+            if (i_f >= Nf) i_f -= Nf;
+            const T wi = (di == 0) ? T(1) : T(0.5);
+            const T r = bf[size_t(jf) * Nf + i_f] - applyL(xf, Nf, i_f, jf, cx[jf], cu[jf], cu[jf - 1]);
+            s += wi * wj * r;
+        }
+    }
+    bc[size_t(jc) * Nc + ic] = s;
+}
+
+// x_f += P x_c (bilinear; coarse rows 0 and Mc+1 are zero).
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_prolong_add(int Nf, int Mf, T *xf, int Nc, const T *xc) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    if (i >= Nf || j > Mf) return;
+    const int ic = i >> 1, jc = j >> 1;
+    const int ic1 = (ic + 1 == Nc) ? 0 : ic + 1;
+    const T *r0 = xc + size_t(jc) * Nc, *r1 = r0 + Nc;
+    T v;
+    if (!(i & 1) && !(j & 1)) v = r0[ic];
+    else if ((i & 1) && !(j & 1)) v = T(0.5) * (r0[ic] + r0[ic1]);
+    else if (!(i & 1)) v = T(0.5) * (r0[ic] + r1[ic]);
+    else v = T(0.25) * (r0[ic] + r0[ic1] + r1[ic] + r1[ic1]);
+    xf[size_t(j) * Nf + i] += v;
+}
+
+// Coarsest level: `iters` damped-Jacobi sweeps from zero in one workgroup,
+// the iterate held in LDS (at most kCoarseMax points).
+constexpr int kCoarseMax = 2048;  // N * (M + 2) points, both buffers in LDS
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_coarse_solve(int N, int M, const T *b, T *x, const T *cx, const T *cu,
+                                                       const T *dinv, T omega, int iters) {
+    __shared__ T xs[2][kCoarseMax];
+    const int n = N * M;
+    const int tot = N * (M + 2);
+    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+        xs[0][t] = 0;
+        xs[1][t] = 0;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int it = 0; it < iters; it++) {
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const int j = 1 + t / N, i = t % N;
+            const size_t c = size_t(j) * N + i;
+            xs[cur ^ 1][c] = xs[cur][c] + omega * dinv[j] * (b[c] - applyL(xs[cur], N, i, j, cx[j], cu[j], cu[j - 1]));
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const size_t c = size_t(1 + t / N) * N + t % N;
+        x[c] = xs[cur][c];
+    }
+}
+
+// ------------------------------------------------------------------ CG
+
+// p = z + beta p (beta = 0 on the first iteration), q = L p, alpha = rz/(p.q).
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_pcg_pq(int N, int M, const T *z, const T *pin, T *pout, T *q, const T *cx,
+                                                   const T *cu, double *sc, RedSlot rs, int first) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (i < N && j <= M) {
+        const T beta = first ? T(0) : T(sc[P_BETA]);
+        const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
+        const size_t r = size_t(j) * N;
+        auto pn = [&](size_t a) -> T { return first ? z[a] : z[a] + beta * pin[a]; };
+        const T c = pn(r + i);
+        const T Lp = cx[j] * (T(2) * c - pn(r + im) - pn(r + ip)) + cu[j] * (c - pn(r + N + i)) +
+                     cu[j - 1] * (c - pn(r - N + i));
+        pout[r + i] = c;
+        q[r + i] = Lp;
+        part[0] = double(c) * double(Lp);
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            sc[P_PQ] = tot[0];
+            sc[P_ALPHA] = sc[P_RZ] / tot[0];
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, first smoothing sweep z = omega D^-1 r, and
+// ||w r||^2 with the row weights w (rhow: the reference system's residual).
+template <typename T>
+__global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r, const T *p, const T *q, T *z,
+                                                       const T *dinv, const T *wD, T omega, double *sc, RedSlot rs) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (i < N && j <= M) {
+        const size_t c = size_t(j) * N + i;
+        const T alpha = T(sc[P_ALPHA]);
+        x[c] = x[c] + alpha * p[c];
+        const T rn = r[c] - alpha * q[c];
+        r[c] = rn;
+        z[c] = omega * dinv[j] * rn;
+        const double wr = double(wD[j]) * double(rn);
+        part[0] = wr * wr;
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot))
+        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_RNORM2] = tot[0];
+}
+
+// r = f - L x and ||w r||^2 into sc[slot] (and sc[slot2] if >= 0).
+template <typename T, typename TF>
+__global__ void __launch_bounds__(LX *LY) k_residual(int N, int M, const TF *f, const T *x, T *r, const T *cx,
+                                                     const T *cu, const T *wD, double *sc, RedSlot rs, int slot,
+                                                     int slot2) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (i < N && j <= M) {
+        const size_t c = size_t(j) * N + i;
+        const T rv = T(f[c]) - applyL(x, N, i, j, cx[j], cu[j], cu[j - 1]);
+        r[c] = rv;
+        const double wr = double(wD[j]) * double(rv);
+        part[0] = wr * wr;
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            sc[slot] = tot[0];
+            if (slot2 >= 0) sc[slot2] = tot[0];
+        }
+    }
+}
+
+// FP64 residual -> FP32 right-hand side of the correction equation, with
+// its weighted norm in the FP32 solver's scalars.
+__global__ void __launch_bounds__(LX *LY) k_to_fp32(int N, int M, const double *r64, float *r32, float *x32,
+                                                    const float *wD, double *sc32, RedSlot rs) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (i < N && j <= M) {
+        const size_t c = size_t(j) * N + i;
+        const float v = float(r64[c]);
+        r32[c] = v;
+        x32[c] = 0.f;
+        const double wr = double(wD[j]) * double(v);
+        part[0] = wr * wr;
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot)) {
+        if (threadIdx.x == 0 && threadIdx.y == 0) {
+            sc32[P_RNORM2] = tot[0];
+            sc32[P_RNORM2_0] = tot[0];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(LX *LY) k_add_correction(int N, int M, double *x64, const float *e32) {
+    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    if (i >= N || j > M) return;
+    const size_t c = size_t(j) * N + i;
+    x64[c] += double(e32[c]);
+}
+
+// ------------------------------------------------------------- cal_w glue
+
+// b = rhow^2 (zetap[i+1] - zetap[i]) dx (src/PoissonSolver.cpp:444-450) for
+// k = 2..nz-2 and f = -b / rhow; ||b||^2 into sc[P_BNORM2].
+__global__ void __launch_bounds__(LX *LY) k_calw_rhs(Grid G, const double *zetap, const double *rhow, double *f,
+                                                     double *sc, RedSlot rs) {
+    const int N = G.nx - 2, M = G.nz - 3;
+    const int ii = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (ii < N && j <= M) {
+        const int i = ii + 1, k = j + 1;
+        const double b = rhow[k] * rhow[k] * (zetap[ix(G, i + 1, k)] - zetap[ix(G, i, k)]) * G.dx;
+        f[size_t(j) * N + ii] = -b / rhow[k];
+        part[0] = b * b;
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot))
+        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_BNORM2] = tot[0];
+}
+
+__global__ void __launch_bounds__(LX *LY) k_rhs_from_b(int N, int M, const double *bref, const double *wD,
+                                                       double *f, double *sc, RedSlot rs) {
+    const int ii = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    double part[1] = {0.};
+    if (ii < N && j <= M) {
+        const double b = bref[size_t(j - 1) * N + ii];
+        f[size_t(j) * N + ii] = -b / wD[j];
+        part[0] = b * b;
+    }
+    double tot[1];
+    if (grid_sum_last<1>(part, rs, tot))
+        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_BNORM2] = tot[0];
+}
+
+__global__ void k_x_ref(int N, int M, double *x, double *xref, int to_ref) {
+    const int ii = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    if (ii >= N || j > M) return;
+    if (to_ref) xref[size_t(j - 1) * N + ii] = x[size_t(j) * N + ii];
+    else x[size_t(j) * N + ii] = xref[size_t(j - 1) * N + ii];
+}
+
+// w[i][k] = x / rhow[k] and BoundaryProcess2D_westdown(w) (:469-476).
+__global__ void __launch_bounds__(LX *LY) k_calw_back(Grid G, const double *x, double *w, const double *rhow) {
+    const int i = blockIdx.x * LX + threadIdx.x, k = blockIdx.y * LY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    if (k == 0 || k == 1 || k == G.nz - 1) {
+        w[ix(G, i, k)] = 0.;
+        return;
+    }
+    if (i < 1 || i > G.nx - 2) return;
+    const int N = G.nx - 2;
+    store_westdown(w, G, i, k, x[size_t(k - 1) * N + (i - 1)] / rhow[k]);
+}
+
+__global__ void k_extrapolate(size_t n, double *x, double *x1, double *x2) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double a = x1[t], b = x2[t];
+    x[t] = 2. * a - b;
+}
+
+// ------------------------------------------------------------------ cal_u
+
+// Sums over 1024 threads of per-thread element runs: exclusive prefix of the
+// thread totals (LDS scratch for 16 waves).
+__device__ inline double block_excl_sum(double v, double *total) {
+    __shared__ double ws[16];
+    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave, nw = blockDim.x / kWave;
+    double inc = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double o = __shfl_up(inc, off, kWave);
+        if (lane >= off) inc += o;
+    }
+    const double below = __shfl_up(inc, 1, kWave);
+    if (lane == kWave - 1) ws[wid] = inc;
+    __syncthreads();
+    double wpre = 0., all = 0.;
+    for (int w2 = 0; w2 < nw; w2++) {
+        if (w2 < wid) wpre += ws[w2];
+        all += ws[w2];
+    }
+    __syncthreads();
+    *total = all;
+    return (lane == 0) ? wpre : wpre + below;
+}
+
+// Top row: h (mean removed), the periodic 1-D problem G y = h solved in
+// closed form through its first differences D_r = y_r - y_{r-1}:
+// D_{r+1} - D_r = h_r and sum_r D_r = 0, so D_r = S_r - mean(S) with S the
+// exclusive prefix sum of h.  uxi = D rdx (mean removed as in the
+// reference), u[i][nz-2] = uxi + ubarTopp, and xi = the mean-free y.
+__global__ void __launch_bounds__(1024) k_calu_top(Grid G, StepPtrs S) {
+    extern __shared__ double sh[];  // N doubles
+    const int N = G.nx - 2, nz = G.nz;
+    const int E = (N + blockDim.x - 1) / blockDim.x;
+    const int r0 = threadIdx.x * E;
+    const double *w = S.w, *rhow = S.P.rhow, *rhou = S.P.rhou;
+    // h and its mean (src/PoissonSolver.cpp:585-594)
+    double loc = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        const double h = -(0. - rhow[nz - 2] * w[ix(G, r + 1, nz - 2)]) / rhou[nz - 2] * G.dx;
+        sh[r] = h;
+        loc += h;
+    }
+    double tot;
+    block_excl_sum(loc, &tot);
+    const double hm = tot / (N);
+    loc = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        sh[r] = sh[r] - hm;
+        loc += sh[r];
+    }
+    // exclusive prefix S_r of h, written over h
+    double pre = block_excl_sum(loc, &tot);
+    double sloc = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        const double h = sh[r];
+        sh[r] = pre;
+        sloc += pre;
+        pre += h;
+    }
+    block_excl_sum(sloc, &tot);
+    const double D0 = -tot / N;
+    // uxi = D rdx, then its mean removed (:621-634)
+    double usum = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        const double D = D0 + sh[r];
+        sh[r] = D;
+        usum += D * G.rdx;
+    }
+    block_excl_sum(usum, &tot);
+    const double um = tot / (N);
+    const double ubp = S.s[S_UBARTOPP];
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        const double uxi = sh[r] * G.rdx - um;
+        S.uxi[r + 1] = uxi;
+        if (r + 1 == N) S.uxi[0] = uxi;
+        if (r == 0) S.uxi[N + 1] = uxi;
+        store_center(S.u, G, r + 1, nz - 2, uxi + ubp);
+    }
+    // xi: y_r = sum_{j<=r} D_j minus its mean (min-norm solution)
+    double dl = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        dl += (r == 0) ? 0. : sh[r];
+    }
+    double ypre = block_excl_sum(dl, &tot);
+    double ysum = 0.;
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        ypre += (r == 0) ? 0. : sh[r];
+        ysum += ypre;
+    }
+    block_excl_sum(ysum, &tot);
+    const double ym = tot / N;
+    ypre = block_excl_sum(dl, &tot);
+    for (int e = 0; e < E; e++) {
+        const int r = r0 + e;
+        if (r >= N) break;
+        ypre += (r == 0) ? 0. : sh[r];
+        const double xi = ypre - ym;
+        S.xi[r + 1] = xi;
+        if (r + 1 == N) S.xi[0] = xi;
+        if (r == 0) S.xi[N + 1] = xi;
+    }
+}
+
+// Downward integration u[i][k] = u[i][nz-2] - dz sum_{k'=k}^{nz-3}
+// ((w[i][k'+1]-w[i-1][k'+1]) rdx - rhow[k'+1] zetap[i][k'+1]) (:643-650):
+// 64 columns x 16 vertical chunks per block, chunk sums scanned in LDS.
+constexpr int UC = 16;
+__global__ void __launch_bounds__(64 * UC) k_calu_cols(Grid G, StepPtrs S) {
+    __shared__ double cs[UC][64];
+    const int i = 1 + blockIdx.x * 64 + threadIdx.x, c = threadIdx.y;
+    const int K = G.nz - 3;  // levels nz-3 .. 1
+    const int C = (K + UC - 1) / UC;
+    const bool col = (i <= G.nx - 2);
+    const double *w = S.w, *zp = S.fut[F_ZETA], *rhow = S.P.rhow;
+    double loc = 0.;
+    if (col) {
+        for (int kk = c * C; kk < min((c + 1) * C, K); kk++) {
+            const int k = G.nz - 3 - kk;
+            loc += ((w[ix(G, i, k + 1)] - w[ix(G, i - 1, k + 1)]) * G.rdx - rhow[k + 1] * zp[ix(G, i, k + 1)]) * -G.dz;
+        }
+    }
+    cs[c][threadIdx.x] = loc;
+    __syncthreads();
+    if (!col) return;
+    double area = 0.;
+    for (int c2 = 0; c2 < c; c2++) area += cs[c2][threadIdx.x];
+    const double utop = S.u[ix(G, i, G.nz - 2)];
+    for (int kk = c * C; kk < min((c + 1) * C, K); kk++) {
+        const int k = G.nz - 3 - kk;
+        area += ((w[ix(G, i, k + 1)] - w[ix(G, i - 1, k + 1)]) * G.rdx - rhow[k + 1] * zp[ix(G, i, k + 1)]) * -G.dz;
+        store_center(S.u, G, i, k, area + utop);
+    }
+}
+
+template <typename T>
+T *dalloc(size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(T)) != hipSuccess) return nullptr;
+    hipMemset(p, 0, n * sizeof(T));
+    return static_cast<T *>(p);
+}
+
+template <typename T>
+bool build_hier(MGHier<T> &H, int N, int M, const std::vector<double> &cx0, const std::vector<double> &cu0) {
+    std::vector<double> cx = cx0, cu = cu0;
+    int l = 0;
+    while (true) {
+        MGLevel<T> &L = H.lev[l];
+        L.N = N;
+        L.M = M;
+        std::vector<T> tcx(M + 2, T(0)), tcu(M + 1), tdi(M + 2, T(0));
+        for (int j = 0; j <= M + 1; j++) tcx[j] = T(cx[j]);
+        for (int j = 0; j <= M; j++) tcu[j] = T(cu[j]);
+        for (int j = 1; j <= M; j++) tdi[j] = T(1. / (2. * cx[j] + cu[j] + cu[j - 1]));
+        L.cx = dalloc<T>(M + 2);
+        L.cu = dalloc<T>(M + 1);
+        L.dinv = dalloc<T>(M + 2);
+        if (!L.cx || !L.cu || !L.dinv) return false;
+        hipMemcpy(L.cx, tcx.data(), sizeof(T) * (M + 2), hipMemcpyHostToDevice);
+        hipMemcpy(L.cu, tcu.data(), sizeof(T) * (M + 1), hipMemcpyHostToDevice);
+        hipMemcpy(L.dinv, tdi.data(), sizeof(T) * (M + 2), hipMemcpyHostToDevice);
+        if (l > 0) {
+            const size_t n = size_t(M + 2) * N;
+            L.x = dalloc<T>(n);
+            L.b = dalloc<T>(n);
+            L.t = dalloc<T>(n);
+            if (!L.x || !L.b || !L.t) return false;
+        }
+        l++;
+        const bool can = (M >= 3) && (M % 2 == 1) && (N >= 4) && (N % 2 == 0) && (l < kMaxLevels);
+        if (!can) break;
+        const int Mc = (M - 1) / 2, Nc = N / 2;
+        std::vector<double> ccx(Mc + 2, 0.), ccu(Mc + 1);
+        for (int jc = 1; jc <= Mc; jc++) ccx[jc] = cx[2 * jc];
+        for (int jc = 0; jc <= Mc; jc++) ccu[jc] = 2. / (1. / cu[2 * jc] + 1. / cu[2 * jc + 1]);
+        cx.swap(ccx);
+        cu.swap(ccu);
+        N = Nc;
+        M = Mc;
+    }
+    H.nlev = l;
+    return true;
+}
+
+template <typename T>
+void free_hier(MGHier<T> &H) {
+    for (int l = 0; l < H.nlev; l++) {
+        MGLevel<T> &L = H.lev[l];
+        hipFree(L.cx);
+        hipFree(L.cu);
+        hipFree(L.dinv);
+        if (l > 0) {
+            hipFree(L.x);
+            hipFree(L.b);
+            hipFree(L.t);
+        }
+    }
+    H.nlev = 0;
+}
+
+// V-cycle on level l >= 1 (x_l from b_l, starting from zero).
+template <typename T>
+void vcycle_coarse(MGHier<T> &H, int l, hipStream_t st) {
+    MGLevel<T> &L = H.lev[l];
+    const dim3 g = lgrid(L.N, L.M), b(LX, LY);
+    if (l == H.nlev - 1) {
+        if (size_t(L.N) * (L.M + 2) <= size_t(kCoarseMax)) {
+            hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), 0, st, L.N, L.M, L.b, L.x, L.cx, L.cu, L.dinv,
+                               H.omega, H.coarse_iters);
+        } else {
+            hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.dinv, H.omega);
+            for (int s = 1; s < 2 * (H.nu1 + H.nu2); s++) {
+                hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
+                std::swap(L.x, L.t);
+            }
+        }
+        return;
+    }
+    hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.dinv, H.omega);
+    for (int s = 1; s < H.nu1; s++) {
+        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
+        std::swap(L.x, L.t);
+    }
+    MGLevel<T> &C = H.lev[l + 1];
+    hipLaunchKernelGGL(k_resid_restrict<T>, lgrid(C.N, C.M), b, 0, st, L.N, L.M, L.b, L.x, L.cx, L.cu, C.N, C.M, C.b);
+    vcycle_coarse(H, l + 1, st);
+    hipLaunchKernelGGL(k_prolong_add<T>, g, b, 0, st, L.N, L.M, L.x, C.N, C.x);
+    for (int s = 0; s < H.nu2; s++) {
+        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
+        std::swap(L.x, L.t);
+    }
+}
+
+// Level-0 V-cycle applied to P.r into P.zA.  The first smoothing sweep has
+// been done already when first_done (fused into k_pcg_update); the last
+// sweep computes r.z and the CG beta.
+template <typename T>
+void vcycle0(PCG<T> &P, bool first_done, bool init, const RedSlot &rs, hipStream_t st) {
+    MGHier<T> &H = P.mg;
+    MGLevel<T> &L = H.lev[0];
+    const dim3 g = lgrid(P.N, P.M), b(LX, LY);
+    if (!first_done) hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, L.dinv, H.omega);
+    for (int s = 1; s < H.nu1; s++) {
+        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega);
+        std::swap(P.zA, P.zB);
+    }
+    if (H.nlev > 1) {
+        MGLevel<T> &C = H.lev[1];
+        hipLaunchKernelGGL(k_resid_restrict<T>, lgrid(C.N, C.M), b, 0, st, P.N, P.M, P.r, P.zA, L.cx, L.cu, C.N, C.M,
+                           C.b);
+        vcycle_coarse(H, 1, st);
+        hipLaunchKernelGGL(k_prolong_add<T>, g, b, 0, st, P.N, P.M, P.zA, C.N, C.x);
+    }
+    for (int s = 0; s < H.nu2; s++) {
+        if (s == H.nu2 - 1)
+            hipLaunchKernelGGL(k_jacobi_dot<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega,
+                               P.sc, rs, int(init));
+        else
+            hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega);
+        std::swap(P.zA, P.zB);
+    }
+}
+
+// Preconditioned CG iterations on P (x, r and sc[P_RNORM2] set up by the
+// caller) until ||w r||^2 <= target or max_iter.  The convergence test lags
+// one V-cycle behind the GPU so the stream never drains.  Returns the
+// iteration count; *rn2 gets the last residual norm^2 seen.
+template <typename T>
+int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
+             hipEvent_t ev, hipStream_t st, double *rn2, bool *conv) {
+    const dim3 g = lgrid(P.N, P.M), b(LX, LY);
+    vcycle0(P, false, true, rs, st);
+    int it = 0;
+    double best = 1e300;
+    int stall = 0;
+    *conv = false;
+    while (it < max_iter) {
+        it++;
+        hipLaunchKernelGGL(k_pcg_pq<T>, g, b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
+                           P.mg.lev[0].cu, P.sc, rs, int(it == 1));
+        std::swap(P.pA, P.pB);
+        hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.zA, P.mg.lev[0].dinv,
+                           P.wD, P.mg.omega, P.sc, rs);
+        hipMemcpyAsync(hs, P.sc + P_RNORM2, sizeof(double), hipMemcpyDeviceToHost, st);
+        hipEventRecord(ev, st);
+        if (it < max_iter) vcycle0(P, true, false, rs, st);
+        hipEventSynchronize(ev);
+        const double rn = *hs;
+        *rn2 = rn;
+        if (!(rn > target)) {
+            *conv = true;
+            break;
+        }
+        if (stall_stop) {
+            if (rn < 0.25 * best) {
+                best = rn;
+                stall = 0;
+            } else if (++stall >= 3) {
+                *conv = true;
+                break;
+            }
+        }
+    }
+    return it;
+}
+
+}  // namespace
+
+int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const RedSlot &rs, bool with_fp32) {
+    N = G.nx - 2;
+    M = G.nz - 3;
+    red = rs;
+    std::vector<double> cx(M + 2, 0.), cu(M + 1, 0.);
+    for (int j = 1; j <= M; j++) cx[j] = 1. / rhow[j + 1];
+    for (int j = 0; j <= M; j++) cu[j] = 1. / rhou[j + 1];
+    const size_t n = size_t(M + 2) * N;
+    std::vector<double> wd(M + 2, 0.);
+    for (int j = 1; j <= M; j++) wd[j] = rhow[j + 1];
+    {
+        PCG<double> &P = pcg64;
+        P.N = N;
+        P.M = M;
+        if (!build_hier(P.mg, N, M, cx, cu)) return VVM_ERR_NOMEM;
+        P.x = dalloc<double>(n);
+        P.r = dalloc<double>(n);
+        P.zA = dalloc<double>(n);
+        P.zB = dalloc<double>(n);
+        P.pA = dalloc<double>(n);
+        P.pB = dalloc<double>(n);
+        P.q = dalloc<double>(n);
+        P.wD = dalloc<double>(M + 2);
+        P.sc = dalloc<double>(NPSCAL);
+        if (!P.x || !P.r || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc) return VVM_ERR_NOMEM;
+        hipMemcpy(P.wD, wd.data(), sizeof(double) * (M + 2), hipMemcpyHostToDevice);
+    }
+    have32 = with_fp32;
+    if (with_fp32) {
+        PCG<float> &P = pcg32;
+        P.N = N;
+        P.M = M;
+        if (!build_hier(P.mg, N, M, cx, cu)) return VVM_ERR_NOMEM;
+        P.x = dalloc<float>(n);
+        P.r = dalloc<float>(n);
+        P.zA = dalloc<float>(n);
+        P.zB = dalloc<float>(n);
+        P.pA = dalloc<float>(n);
+        P.pB = dalloc<float>(n);
+        P.q = dalloc<float>(n);
+        P.wD = dalloc<float>(M + 2);
+        P.sc = dalloc<double>(NPSCAL);
+        if (!P.x || !P.r || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc) return VVM_ERR_NOMEM;
+        std::vector<float> wf(wd.begin(), wd.end());
+        hipMemcpy(P.wD, wf.data(), sizeof(float) * (M + 2), hipMemcpyHostToDevice);
+        r64 = dalloc<double>(n);
+        if (!r64) return VVM_ERR_NOMEM;
+    }
+    f = dalloc<double>(n);
+    xprev = dalloc<double>(n);
+    xprev2 = dalloc<double>(n);
+    if (!f || !xprev || !xprev2) return VVM_ERR_NOMEM;
+    if (hipHostMalloc(reinterpret_cast<void **>(&hscal), 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return VVM_ERR_NOMEM;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return VVM_ERR_HIP;
+    return VVM_OK;
+}
+
+template <typename T>
+static void free_pcg(PCG<T> &P) {
+    free_hier(P.mg);
+    for (void *p : {(void *)P.x, (void *)P.r, (void *)P.zA, (void *)P.zB, (void *)P.pA, (void *)P.pB, (void *)P.q,
+                    (void *)P.wD, (void *)P.sc})
+        if (p) hipFree(p);
+    P = PCG<T>();
+}
+
+void Poisson::destroy() {
+    free_pcg(pcg64);
+    if (have32) free_pcg(pcg32);
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64})
+        if (p) hipFree(p);
+    f = xprev = xprev2 = r64 = nullptr;
+    if (hscal) hipHostFree(hscal);
+    hscal = nullptr;
+    if (ev) hipEventDestroy(ev);
+    ev = nullptr;
+}
+
+void Poisson::rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st) {
+    hipLaunchKernelGGL(k_calw_rhs, lgrid(N, M), dim3(LX, LY), 0, st, G, zetap, rhow_dev, f, pcg64.sc, red);
+}
+
+void Poisson::rhs_from_b(const double *b, hipStream_t st) {
+    // the FP64 row weights wD[j] are rhow[j+1]
+    hipLaunchKernelGGL(k_rhs_from_b, lgrid(N, M), dim3(LX, LY), 0, st, N, M, b, pcg64.wD, f, pcg64.sc, red);
+}
+
+void Poisson::x_to_ref(double *xref, hipStream_t st) {
+    hipLaunchKernelGGL(k_x_ref, lgrid(N, M), dim3(LX, LY), 0, st, N, M, pcg64.x, xref, 1);
+}
+
+void Poisson::x_from_ref(const double *xref, hipStream_t st) {
+    hipLaunchKernelGGL(k_x_ref, lgrid(N, M), dim3(LX, LY), 0, st, N, M, pcg64.x, const_cast<double *>(xref), 0);
+}
+
+void Poisson::back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStream_t st) {
+    hipLaunchKernelGGL(k_calw_back, dim3((G.nx + LX - 1) / LX, (G.nz + LY - 1) / LY), dim3(LX, LY), 0, st, G, pcg64.x,
+                       w, rhow_dev);
+}
+
+void Poisson::set_guess(int mode, hipStream_t st) {
+    const size_t n = size_t(M + 2) * N;
+    if (mode == VVM_GUESS_ZERO) {
+        hipMemsetAsync(pcg64.x, 0, n * sizeof(double), st);
+    } else if (mode == VVM_GUESS_EXTRAP && nsolves >= 2) {
+        hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, n, pcg64.x, xprev, xprev2);
+    }
+    // VVM_GUESS_PREV (and the learned guess, written by its own kernels)
+    // start from the current contents of x.
+}
+
+void Poisson::remember_solution(hipStream_t st) {
+    const size_t n = size_t(M + 2) * N;
+    std::swap(xprev, xprev2);
+    hipMemcpyAsync(xprev, pcg64.x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    nsolves++;
+}
+
+SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
+    SolveResult R;
+    PCG<double> &P = pcg64;
+    const dim3 g = lgrid(N, M), b(LX, LY);
+    hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, P.x, P.r, P.mg.lev[0].cx, P.mg.lev[0].cu,
+                       P.wD, P.sc, red, int(P_RNORM2), int(P_RNORM2_0));
+    hipMemcpyAsync(hscal, P.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
+    if (bn2 == 0.) {
+        hipMemsetAsync(P.x, 0, size_t(M + 2) * N * sizeof(double), st);
+        R.converged = true;
+        return R;
+    }
+    R.rel0 = std::sqrt(rn2 / bn2);
+    const bool parity = (tol <= 0.);
+    const double target = parity ? 1e-30 * bn2 : tol * tol * bn2;
+    if (!(rn2 > target)) {
+        R.rel = R.rel0;
+        R.converged = true;
+        return R;
+    }
+    double last = rn2;
+    bool conv = false;
+    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, ev, st, &last, &conv);
+    R.rel = std::sqrt(last / bn2);
+    R.converged = conv;
+    return R;
+}
+
+SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, double inner_tol, hipStream_t st) {
+    SolveResult R;
+    if (!have32) return solve_fp64(tol, max_inner * max_outer, st);
+    PCG<double> &D = pcg64;
+    PCG<float> &F = pcg32;
+    const dim3 g = lgrid(N, M), b(LX, LY);
+    double bn2 = -1.;
+    for (int outer = 0; outer <= max_outer; outer++) {
+        hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, D.x, r64, D.mg.lev[0].cx,
+                           D.mg.lev[0].cu, D.wD, D.sc, red, int(P_RNORM2), -1);
+        hipMemcpyAsync(hscal, D.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        bn2 = hscal[P_BNORM2];
+        const double rn2 = hscal[P_RNORM2];
+        if (bn2 == 0.) {
+            hipMemsetAsync(D.x, 0, size_t(M + 2) * N * sizeof(double), st);
+            R.converged = true;
+            return R;
+        }
+        const double rel = std::sqrt(rn2 / bn2);
+        if (outer == 0) R.rel0 = rel;
+        R.rel = rel;
+        if (rel <= tol) {
+            R.converged = true;
+            return R;
+        }
+        if (outer == max_outer) break;
+        R.refine++;
+        hipLaunchKernelGGL(k_to_fp32, g, b, 0, st, N, M, r64, F.r, F.x, F.wD, F.sc, red);
+        hipMemcpyAsync(hscal, F.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
+        hipStreamSynchronize(st);
+        const double r0 = hscal[P_RNORM2];
+        double last = r0;
+        bool conv = false;
+        R.iters += pcg_loop(F, inner_tol * inner_tol * r0, max_inner, false, red, hscal + 8, ev, st, &last, &conv);
+        hipLaunchKernelGGL(k_add_correction, g, b, 0, st, N, M, D.x, F.x);
+    }
+    return R;
+}
+
+// ---------------------------------------------------------------- cal_u
+
+bool cal_u_supported(const Grid &G) { return G.nx - 2 <= 8192; }
+
+void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st) {
+    const int N = G.nx - 2;
+    hipLaunchKernelGGL(k_calu_top, dim3(1), dim3(1024), size_t(N) * sizeof(double), st, G, S);
+    hipLaunchKernelGGL(k_calu_cols, dim3((N + 63) / 64), dim3(64, UC), 0, st, G, S);
+}
+
+}  // namespace vvm

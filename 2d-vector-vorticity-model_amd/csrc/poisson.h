@@ -1,0 +1,110 @@
+// poisson.h -- the cal_w elliptic solve on the device.
+//
+// The reference assembles A (src/PoissonSolver.cpp:760-796) for the unknown
+// x = rhow[k] w[i][k], i = 1..nx-2 (periodic), k = 2..nz-2, with Dirichlet
+// zeros at k = 1 and k = nz-1.  Row (i,k) of A divided by rhow[k] is
+//   -L x = cx_k (x_{i-1} - 2 x_i + x_{i+1}) + cu_k (x_{k+1} - x_k) - cu_{k-1} (x_k - x_{k-1})
+// with cx_k = 1/rhow[k] and cu_k = 1/rhou[k]: L is symmetric positive
+// definite, so the solve is preconditioned CG on L x = f, f = -b / rhow.
+// The true residual of the reference system is ||b - A x|| = ||rhow r||.
+//
+// Unknown arrays are (M+2) rows x N columns, x-contiguous, rows 0 and M+1
+// held at zero (the Dirichlet rows); N = nx-2, M = nz-3.
+#pragma once
+
+#include "vvm_internal.h"
+
+namespace vvm {
+
+template <typename T>
+struct MGLevel {
+    int N = 0, M = 0;
+    T *cx = nullptr, *cu = nullptr, *dinv = nullptr;  // [M+2], [M+1], [M+2]
+    T *x = nullptr, *b = nullptr, *t = nullptr;       // (M+2)*N each
+};
+
+constexpr int kMaxLevels = 16;
+
+template <typename T>
+struct MGHier {
+    int nlev = 0;
+    MGLevel<T> lev[kMaxLevels];
+    T omega = T(0.8);
+    int nu1 = 2, nu2 = 2;
+    int coarse_iters = 64;
+};
+
+// PCG device scalars (doubles).
+enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, NPSCAL = 8 };
+
+// One CG instance in precision T (vectors on level 0 of its hierarchy).
+template <typename T>
+struct PCG {
+    int N = 0, M = 0;
+    MGHier<T> mg;
+    T *x = nullptr, *r = nullptr, *zA = nullptr, *zB = nullptr, *pA = nullptr, *pB = nullptr, *q = nullptr;
+    T *wD = nullptr;       // row weights rhow[k] for ||D r||, [M+2]
+    double *sc = nullptr;  // NPSCAL device scalars
+};
+
+struct SolveResult {
+    int iters = 0;
+    int refine = 0;
+    double rel = 0., rel0 = 0.;
+    bool converged = false;
+};
+
+class Poisson {
+   public:
+    // rhou/rhow are the host profiles (nz long).  Allocates with hipMalloc.
+    int init(const Grid &G, const double *rhou, const double *rhow, const RedSlot &rs, bool with_fp32);
+    void destroy();
+
+    // cal_w right-hand side: f from zetap, and ||b||^2.
+    void rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st);
+    // b given in the reference's row order (device pointer); fills f and ||b||^2.
+    void rhs_from_b(const double *b, hipStream_t st);
+    // x (the reference's row order, device) <-> internal solution array.
+    void x_to_ref(double *xref, hipStream_t st);
+    void x_from_ref(const double *xref, hipStream_t st);
+    // w = x / rhow and the westdown halo.
+    void back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStream_t st);
+
+    // Guess handling: VVM_GUESS_ZERO clears x, VVM_GUESS_EXTRAP forms
+    // 2 x_{t-1} - x_{t-2}; the others keep the current contents of x.
+    void set_guess(int mode, hipStream_t st);
+    // Record x as the latest solution (for the extrapolated guess).
+    void remember_solution(hipStream_t st);
+    double *xprev_ptr() const { return xprev; }
+    double *xprev2_ptr() const { return xprev2; }
+    int nsolves = 0;
+    // Plain FP64 MG-PCG.  tol: relative true residual ||b-Ax||/||b||.
+    SolveResult solve_fp64(double tol, int max_iter, hipStream_t st);
+    // FP32 MG-PCG inside FP64 iterative refinement.
+    SolveResult solve_mixed(double tol, int max_outer, int max_inner, double inner_tol, hipStream_t st);
+
+    double *x64() const { return pcg64.x; }
+    double *f64() const { return f; }
+    int N = 0, M = 0;
+
+   private:
+    void vcycle64(hipStream_t st, bool first_done, double **zcur);
+    SolveResult pcg_run64(double tol, int max_iter, hipStream_t st);
+    PCG<double> pcg64;
+    PCG<float> pcg32;
+    bool have32 = false;
+    double *f = nullptr;       // rhs of L x = f, (M+2)*N
+    double *xprev = nullptr;   // previous solution (for extrapolation)
+    double *xprev2 = nullptr;
+    double *r64 = nullptr;     // FP64 residual of the refinement loop
+    double *hscal = nullptr;   // pinned host copy of scalars
+    RedSlot red;
+    hipEvent_t ev = nullptr;
+};
+
+// cal_u on the device (src/PoissonSolver.cpp:580-653): top-row potential
+// flow and the downward integration of u.
+bool cal_u_supported(const Grid &G);  // nx - 2 <= 8192
+void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st);
+
+}  // namespace vvm

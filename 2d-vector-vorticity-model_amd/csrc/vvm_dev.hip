@@ -1,0 +1,778 @@
+// vvm_dev.hip -- the C ABI of include/vvm_dev.h: device state of one model,
+// upload/download with the one-time [i][k] <-> [k][i] transpose, and the
+// timestep in the order of vvm::Iteration::TimeMarching
+// (src/Iteration.cpp:227-436).
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "poisson.h"
+#include "vvm_internal.h"
+
+using namespace vvm;
+
+struct vvm_dev {
+    int device = 0;
+    hipStream_t st = nullptr;
+    vvm_dev_config cfg{};
+    Grid G{};
+    std::string err;
+
+    std::vector<void *> allocs;
+    double *prog[NPROG][3] = {};
+    int idx_now = 0, idx_past = 1, idx_fut = 2;
+    double *hist[NHIST][2] = {};
+    double *u = nullptr, *w = nullptr, *RKM = nullptr, *RKH = nullptr, *forcing = nullptr;
+    double *evap = nullptr, *accr = nullptr, *autoc = nullptr, *cond = nullptr;
+    double *precip = nullptr, *xi = nullptr, *uxi = nullptr;
+    double *s = nullptr;
+    Profiles P{};
+    Reductions red{};
+
+    Poisson pois;
+    bool pois_ready = false;
+    std::vector<double> h_rhou, h_rhow;
+
+    int step = 0;
+    int status_for_adding_forcing = 0;
+    bool uploaded = false;
+    bool ghosts_dirty = true;
+    bool stepped = false;
+    // After a rotation the reference's p arrays hold copies of now
+    // (src/Iteration.cpp:161 copies p into now and leaves p as is); the
+    // rotated p slot holds stale data the next step overwrites unread.
+    bool p_alias_now = false;
+    std::vector<double> h_thb_zeta;  // as uploaded (the step does not read it)
+
+    double *pinned = nullptr;  // nx*nz staging for the random kick
+    size_t pinned_n = 0;
+    hipEvent_t ev[8] = {};
+
+    // per-kernel event timer (vvm_dev_kernel_timer)
+    static constexpr int kKtMax = 256;
+    int kt_kernel = 0;
+    std::vector<hipEvent_t> kt_ev;
+    int kt_used = 0;
+    double kt_ms = 0.;
+    long kt_n = 0;
+};
+
+namespace {
+// Brackets one kernel launch (or the solve) with events when it is the
+// kernel selected by vvm_dev_kernel_timer.
+struct KTimer {
+    vvm_dev *h;
+    bool on;
+    KTimer(vvm_dev *hh, int id) : h(hh), on(hh->kt_kernel == id && hh->kt_used < vvm_dev::kKtMax) {
+        if (on) hipEventRecord(h->kt_ev[2 * h->kt_used], h->st);
+    }
+    ~KTimer() {
+        if (on) {
+            hipEventRecord(h->kt_ev[2 * h->kt_used + 1], h->st);
+            h->kt_used++;
+        }
+    }
+};
+
+void kt_collect(vvm_dev *h) {
+    for (int j = 0; j < h->kt_used; j++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->kt_ev[2 * j], h->kt_ev[2 * j + 1]) == hipSuccess) {
+            h->kt_ms += ms;
+            h->kt_n++;
+        }
+    }
+    h->kt_used = 0;
+}
+}  // namespace
+
+namespace {
+
+int fail(vvm_dev *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHK(h, expr)                                                                        \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail((h), VVM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+    } while (0)
+
+double *dev_alloc(vvm_dev *h, size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, n * sizeof(double)) != hipSuccess) return nullptr;
+    hipMemset(p, 0, n * sizeof(double));
+    h->allocs.push_back(p);
+    return static_cast<double *>(p);
+}
+
+StepPtrs ptrs(vvm_dev *h) {
+    StepPtrs S{};
+    for (int f = 0; f < NPROG; f++) {
+        S.now[f] = h->prog[f][h->idx_now];
+        S.past[f] = h->prog[f][h->idx_past];
+        S.fut[f] = h->prog[f][h->idx_fut];
+    }
+    for (int q = 0; q < NHIST; q++) {
+        S.hist[q][0] = h->hist[q][0];
+        S.hist[q][1] = h->hist[q][1];
+    }
+    S.u = h->u;
+    S.w = h->w;
+    S.RKM = h->RKM;
+    S.RKH = h->RKH;
+    S.forcing = h->forcing;
+    S.evap = h->evap;
+    S.accr = h->accr;
+    S.autoc = h->autoc;
+    S.cond = h->cond;
+    S.precip = h->precip;
+    S.xi = h->xi;
+    S.uxi = h->uxi;
+    S.s = h->s;
+    S.P = h->P;
+    return S;
+}
+
+// m <- now <- p by rotation (src/Iteration.cpp:158-203 copies).
+void rotate(vvm_dev *h) {
+    const int past = h->idx_past, now = h->idx_now, fut = h->idx_fut;
+    h->idx_past = now;
+    h->idx_now = fut;
+    h->idx_fut = past;
+    h->p_alias_now = true;
+}
+
+// Host [i][k] (k fastest) -> device [k][i] with pitch, and back.
+void to_dev_layout(const Grid &G, const double *src, std::vector<double> &dst, int stride = 1, int off = 0) {
+    dst.assign(size_t(G.nz) * G.pitch, 0.);
+    for (int i = 0; i < G.nx; i++)
+        for (int k = 0; k < G.nz; k++) dst[size_t(k) * G.pitch + i] = src[(size_t(i) * G.nz + k) * stride + off];
+}
+void from_dev_layout(const Grid &G, const std::vector<double> &src, double *dst, int stride = 1, int off = 0) {
+    for (int i = 0; i < G.nx; i++)
+        for (int k = 0; k < G.nz; k++) dst[(size_t(i) * G.nz + k) * stride + off] = src[size_t(k) * G.pitch + i];
+}
+
+int put2d(vvm_dev *h, double *d, const double *src, int stride = 1, int off = 0) {
+    std::vector<double> tmp;
+    to_dev_layout(h->G, src, tmp, stride, off);
+    HIPCHK(h, hipMemcpy(d, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice));
+    return VVM_OK;
+}
+int get2d(vvm_dev *h, const double *d, double *dst, int stride = 1, int off = 0) {
+    std::vector<double> tmp(size_t(h->G.nz) * h->G.pitch);
+    HIPCHK(h, hipMemcpy(tmp.data(), d, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    from_dev_layout(h->G, tmp, dst, stride, off);
+    return VVM_OK;
+}
+
+// Per-level constants the kernels read instead of re-evaluating pow()
+// (host C library, the reference's own rounding).
+int upload_level_constants(vvm_dev *h, const vvm_host_view *v) {
+    const vvm_dev_config &c = h->cfg;
+    const int nz = c.nz;
+    std::vector<double> vtfac(nz, 0.), pcoef(nz, 0.), rth(nz, 0.), rz(nz, 0.);
+    for (int k = 0; k < nz; k++) {
+        vtfac[k] = std::pow(v->rhow[k] / v->rhow[1], -0.5);
+        pcoef[k] = 380. / (std::pow(v->pib[k], c.Cp / c.Rd) * c.P0);
+    }
+    // Nudge_theta / Nudge_zeta rates dt*CGR (NumericalProcess.cpp:60-84).
+    const int ks = int(12000. / c.dz + 1);
+    const int kr = (nz - 1) - ks;
+    const double crad = 1. / 3600.;
+    if (kr >= 0) {
+        for (int k = ks; k < nz - 1; k++) {
+            const double cgr = crad * (v->z[k] - v->z[kr]) / (v->z[nz - 1] - v->z[kr]);
+            rth[k] = c.dt * cgr;
+            const double cgz = (1. / 360.) * (v->z_zeta[k] - v->z_zeta[kr]) / (v->z_zeta[nz - 1] - v->z_zeta[kr]);
+            rz[k] = (c.CASE != 2) ? c.dt * cgz : 0.;
+        }
+    }
+    HIPCHK(h, hipMemcpy(h->P.vtfac, vtfac.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->P.pcoef, pcoef.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->P.rate_th, rth.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(h, hipMemcpy(h->P.rate_z, rz.data(), nz * sizeof(double), hipMemcpyHostToDevice));
+    const double pf = std::pow(v->rhou[1] / v->rhow[1], -0.5);
+    HIPCHK(h, hipMemcpy(h->s + S_PRECIP_FAC, &pf, sizeof(double), hipMemcpyHostToDevice));
+    return VVM_OK;
+}
+
+int ensure_poisson(vvm_dev *h, const double *rhou, const double *rhow) {
+    const int nz = h->cfg.nz;
+    std::vector<double> ru(rhou, rhou + nz), rw(rhow, rhow + nz);
+    if (h->pois_ready && ru == h->h_rhou && rw == h->h_rhow) return VVM_OK;
+    if (h->pois_ready) h->pois.destroy();
+    h->pois_ready = false;
+    const int rc = h->pois.init(h->G, rhou, rhow, h->red.solver, true);
+    if (rc != VVM_OK) return fail(h, rc, "Poisson solver allocation failed");
+    h->pois_ready = true;
+    h->h_rhou = ru;
+    h->h_rhow = rw;
+    return VVM_OK;
+}
+
+int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss) {
+    const vvm_dev_config &c = h->cfg;
+    h->pois.rhs_from_zetap(h->G, S.fut[F_ZETA], h->P.rhow, h->st);
+    h->pois.set_guess(c.guess_mode, h->st);
+    SolveResult R;
+    const int maxit = c.max_iter > 0 ? c.max_iter : 200;
+    if (c.solver_mode == VVM_SOLVER_MIXED) {
+        const double tol = (c.stop_mode == VVM_STOP_PARITY) ? 1e-13 : c.tol_rel;
+        R = h->pois.solve_mixed(tol, 8, maxit, 1e-4, h->st);
+    } else {
+        const double tol = (c.stop_mode == VVM_STOP_PARITY) ? 0. : c.tol_rel;
+        R = h->pois.solve_fp64(tol, maxit, h->st);
+    }
+    h->pois.remember_solution(h->st);
+    h->pois.back_to_w(h->G, S.w, h->P.rhow, h->st);
+    if (ss) {
+        ss->iters = R.iters;
+        ss->refine = R.refine;
+        ss->rel_residual = R.rel;
+        ss->rel_residual0 = R.rel0;
+    }
+    return R.converged ? VVM_OK : VVM_WARN_NOT_CONVERGED;
+}
+
+int check_launch(vvm_dev *h) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(h, VVM_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    return VVM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
+    if (!cfg || !out) return VVM_ERR_ARG;
+    *out = nullptr;
+    if (cfg->nx < 8 || cfg->nz < 8 || cfg->dx <= 0 || cfg->dz <= 0 || cfg->dt <= 0) return VVM_ERR_SIZE;
+    vvm_dev *h = new vvm_dev();
+    h->cfg = *cfg;
+    h->device = device;
+    Grid &G = h->G;
+    G.nx = cfg->nx;
+    G.nz = cfg->nz;
+    G.pitch = (cfg->nx + 63) / 64 * 64;
+    G.dx = cfg->dx;
+    G.dz = cfg->dz;
+    G.dt = cfg->dt;
+    // vvm constructor (src/Declare.hpp:79-84)
+    G.rdx = 1.0 / cfg->dx;
+    G.r2dx = G.rdx / 2.0;
+    G.rdz = 1.0 / cfg->dz;
+    G.r2dz = G.rdz / 2.0;
+    G.rdx2 = G.rdx * G.rdx;
+    G.rdz2 = G.rdz * G.rdz;
+    G.GRAVITY = cfg->GRAVITY;
+    G.Cp = cfg->Cp;
+    G.Cv = cfg->Cv;
+    G.Rd = cfg->Rd;
+    G.Lv = cfg->Lv;
+    G.P0 = cfg->P0;
+    if (!diffusion_supported(G) || !cal_u_supported(G)) {
+        delete h;
+        return VVM_ERR_SIZE;
+    }
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return VVM_ERR_HIP;
+    }
+    const size_t n2 = size_t(G.nz) * G.pitch;
+    bool ok = true;
+    auto A = [&](size_t n) {
+        double *p = dev_alloc(h, n);
+        ok = ok && p;
+        return p;
+    };
+    for (int f = 0; f < NPROG; f++)
+        for (int s = 0; s < 3; s++) h->prog[f][s] = A(n2);
+    for (int q = 0; q < NHIST; q++)
+        for (int s = 0; s < 2; s++) h->hist[q][s] = A(n2);
+    h->u = A(n2);
+    h->w = A(n2);
+    h->RKM = A(n2);
+    h->RKH = A(n2);
+    h->forcing = A(n2);
+    h->evap = A(n2);
+    h->accr = A(n2);
+    h->autoc = A(n2);
+    h->cond = A(n2);
+    h->precip = A(G.nx);
+    h->xi = A(G.nx);
+    h->uxi = A(G.nx);
+    h->s = A(NSCALAR);
+    double **prof[] = {&h->P.rhou, &h->P.rhow, &h->P.pib, &h->P.pb, &h->P.thb, &h->P.thbm, &h->P.thb_init,
+                       &h->P.thb_zeta, &h->P.thvb, &h->P.thvbm, &h->P.qvb, &h->P.qvb0, &h->P.qvsb, &h->P.z,
+                       &h->P.z_zeta, &h->P.lambda2, &h->P.Q1LS, &h->P.Q2LS, &h->P.vtfac, &h->P.pcoef,
+                       &h->P.rate_th, &h->P.rate_z};
+    for (double **p : prof) *p = A(G.nz);
+    // reduction scratch: enough partials for the largest launch grid
+    const int maxb = ((G.nx + 63) / 64 + 1) * ((G.nz + 3) / 4 + 1);
+    RedSlot *slots[] = {&h->red.negval, &h->red.solver, &h->red.misc};
+    for (RedSlot *rs : slots) {
+        rs->max_blocks = maxb;
+        rs->partial = A(size_t(8) * maxb);
+        rs->counter = reinterpret_cast<unsigned *>(A(1));
+    }
+    for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    if (!ok) {
+        vvm_dev_destroy(h);
+        return VVM_ERR_NOMEM;
+    }
+    *out = h;
+    return VVM_OK;
+}
+
+void vvm_dev_destroy(vvm_dev_t *h) {
+    if (!h) return;
+    hipSetDevice(h->device);
+    if (h->st) hipStreamSynchronize(h->st);
+    if (h->pois_ready) h->pois.destroy();
+    for (void *p : h->allocs) hipFree(p);
+    if (h->pinned) hipHostFree(h->pinned);
+    for (auto &e : h->ev)
+        if (e) hipEventDestroy(e);
+    for (auto &e : h->kt_ev)
+        if (e) hipEventDestroy(e);
+    if (h->st) hipStreamDestroy(h->st);
+    delete h;
+}
+
+const char *vvm_dev_last_error(const vvm_dev_t *h) { return h ? h->err.c_str() : "null handle"; }
+
+int vvm_dev_upload(vvm_dev_t *h, const vvm_host_view *v) {
+    if (!h || !v) return VVM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    h->idx_now = 0;
+    h->idx_past = 1;
+    h->idx_fut = 2;
+    const double *trip[NPROG][3] = {{v->zeta, v->zetam, v->zetap}, {v->th, v->thm, v->thp}, {v->qv, v->qvm, v->qvp},
+                                    {v->qc, v->qcm, v->qcp},       {v->qr, v->qrm, v->qrp}};
+    for (int f = 0; f < NPROG; f++)
+        for (int s = 0; s < 3; s++) {
+            if (!trip[f][s]) return fail(h, VVM_ERR_ARG, "upload: prognostic field missing");
+            int rc = put2d(h, h->prog[f][s], trip[f][s]);
+            if (rc) return rc;
+        }
+    const double *hv[NHIST] = {v->dth_advect, v->dth_buoyancy, v->dzeta_advect, v->dqv_advect,
+                               v->dqc_advect, v->dqr_advect,   v->dqr_VT};
+    for (int q = 0; q < NHIST; q++) {
+        if (!hv[q]) return fail(h, VVM_ERR_ARG, "upload: AB2 history missing");
+        for (int s = 0; s < 2; s++) {
+            int rc = put2d(h, h->hist[q][s], hv[q], 2, s);
+            if (rc) return rc;
+        }
+    }
+    struct {
+        const double *src;
+        double *dst;
+        bool required;
+    } f2[] = {{v->u, h->u, true},          {v->w, h->w, true},           {v->RKM, h->RKM, true},
+              {v->RKH, h->RKH, true},      {v->init_th_forcing, h->forcing, false},
+              {v->evaporation, h->evap, false}, {v->accretion, h->accr, false},
+              {v->autoconversion, h->autoc, false}, {v->condensation, h->cond, false}};
+    for (auto &e : f2) {
+        if (!e.src) {
+            if (e.required) return fail(h, VVM_ERR_ARG, "upload: u/w/RKM/RKH missing");
+            continue;
+        }
+        int rc = put2d(h, e.dst, e.src);
+        if (rc) return rc;
+    }
+    const int nx = h->G.nx, nz = h->G.nz;
+    struct {
+        const double *src;
+        double *dst;
+        int n;
+    } f1[] = {{v->precip, h->precip, nx}, {v->xi, h->xi, nx},          {v->uxi, h->uxi, nx},
+              {v->rhou, h->P.rhou, nz},   {v->rhow, h->P.rhow, nz},    {v->pib, h->P.pib, nz},
+              {v->pb, h->P.pb, nz},       {v->thb, h->P.thb, nz},      {v->thbm, h->P.thbm, nz},
+              {v->thb_init, h->P.thb_init, nz}, {v->thb_zeta, h->P.thb_zeta, nz}, {v->thvb, h->P.thvb, nz},
+              {v->thvbm, h->P.thvbm, nz}, {v->qvb, h->P.qvb, nz},      {v->qvb0, h->P.qvb0, nz},
+              {v->qvsb, h->P.qvsb, nz},   {v->z, h->P.z, nz},          {v->z_zeta, h->P.z_zeta, nz},
+              {v->lambda2, h->P.lambda2, nz}, {v->Q1LS, h->P.Q1LS, nz}, {v->Q2LS, h->P.Q2LS, nz}};
+    for (auto &e : f1) {
+        if (!e.src) {
+            if (e.dst == h->precip || e.dst == h->xi || e.dst == h->uxi || e.dst == h->P.thbm ||
+                e.dst == h->P.thvbm || e.dst == h->P.qvsb || e.dst == h->P.thb_zeta)
+                continue;
+            return fail(h, VVM_ERR_ARG, "upload: profile missing");
+        }
+        HIPCHK(h, hipMemcpy(e.dst, e.src, e.n * sizeof(double), hipMemcpyHostToDevice));
+    }
+    if (v->thb_zeta) h->h_thb_zeta.assign(v->thb_zeta, v->thb_zeta + nz);
+    double sc[NSCALAR] = {0.};
+    sc[S_UBARTOP] = v->ubarTop;
+    sc[S_UBARTOPP] = v->ubarTopp;
+    sc[S_UBARTOPM] = v->ubarTopm;
+    sc[S_DUBAR0] = v->dubarTop_advect[0];
+    sc[S_DUBAR1] = v->dubarTop_advect[1];
+    HIPCHK(h, hipMemcpy(h->s, sc, S_NEG_RATIO * sizeof(double), hipMemcpyHostToDevice));
+    int rc = upload_level_constants(h, v);
+    if (rc) return rc;
+    rc = ensure_poisson(h, v->rhou, v->rhow);
+    if (rc) return rc;
+    h->step = v->step;
+    h->status_for_adding_forcing = v->status_for_adding_forcing;
+    h->uploaded = true;
+    h->ghosts_dirty = true;
+    h->stepped = false;
+    h->p_alias_now = false;
+    return VVM_OK;
+}
+
+int vvm_dev_download(vvm_dev_t *h, vvm_host_view *v) {
+    if (!h || !v) return VVM_ERR_ARG;
+    if (!h->uploaded) return fail(h, VVM_ERR_STATE, "download before upload");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    double *trip[NPROG][3] = {{v->zeta, v->zetam, v->zetap}, {v->th, v->thm, v->thp}, {v->qv, v->qvm, v->qvp},
+                              {v->qc, v->qcm, v->qcp},       {v->qr, v->qrm, v->qrp}};
+    const int order[3] = {h->idx_now, h->idx_past, h->p_alias_now ? h->idx_now : h->idx_fut};
+    for (int f = 0; f < NPROG; f++)
+        for (int s = 0; s < 3; s++)
+            if (trip[f][s]) {
+                int rc = get2d(h, h->prog[f][order[s]], trip[f][s]);
+                if (rc) return rc;
+            }
+    double *hv[NHIST] = {v->dth_advect, v->dth_buoyancy, v->dzeta_advect, v->dqv_advect,
+                         v->dqc_advect, v->dqr_advect,   v->dqr_VT};
+    for (int q = 0; q < NHIST; q++)
+        if (hv[q])
+            for (int s = 0; s < 2; s++) {
+                int rc = get2d(h, h->hist[q][s], hv[q], 2, s);
+                if (rc) return rc;
+            }
+    struct {
+        double *dst;
+        const double *src;
+    } f2[] = {{v->u, h->u},           {v->w, h->w},           {v->RKM, h->RKM},         {v->RKH, h->RKH},
+              {v->init_th_forcing, h->forcing}, {v->evaporation, h->evap}, {v->accretion, h->accr},
+              {v->autoconversion, h->autoc}, {v->condensation, h->cond}};
+    for (auto &e : f2)
+        if (e.dst) {
+            int rc = get2d(h, e.src, e.dst);
+            if (rc) return rc;
+        }
+    const int nx = h->G.nx, nz = h->G.nz;
+    struct {
+        double *dst;
+        const double *src;
+        int n;
+    } f1[] = {{v->precip, h->precip, nx}, {v->xi, h->xi, nx},          {v->uxi, h->uxi, nx},
+              {v->rhou, h->P.rhou, nz},   {v->rhow, h->P.rhow, nz},    {v->pib, h->P.pib, nz},
+              {v->pb, h->P.pb, nz},       {v->thb, h->P.thb, nz},      {v->thbm, h->P.thbm, nz},
+              {v->thb_init, h->P.thb_init, nz}, {v->thvb, h->P.thvb, nz}, {v->thvbm, h->P.thvbm, nz},
+              {v->qvb, h->P.qvb, nz},     {v->qvb0, h->P.qvb0, nz},    {v->qvsb, h->P.qvsb, nz},
+              {v->z, h->P.z, nz},         {v->z_zeta, h->P.z_zeta, nz}, {v->lambda2, h->P.lambda2, nz},
+              {v->Q1LS, h->P.Q1LS, nz},   {v->Q2LS, h->P.Q2LS, nz}};
+    for (auto &e : f1)
+        if (e.dst) HIPCHK(h, hipMemcpy(e.dst, e.src, e.n * sizeof(double), hipMemcpyDeviceToHost));
+    if (v->thb_zeta) {
+        if (h->stepped && v->thb) {
+            // updateMean's thb_zeta (src/Iteration.cpp:80-84), not read by the step
+            for (int k = 1; k < nz - 1; k++) v->thb_zeta[k] = 0.5 * (v->thb[k - 1] + v->thb[k]);
+            v->thb_zeta[0] = v->thb_zeta[1];
+            v->thb_zeta[nz - 1] = v->thb_zeta[nz - 2];
+        } else if (!h->h_thb_zeta.empty()) {
+            std::copy(h->h_thb_zeta.begin(), h->h_thb_zeta.end(), v->thb_zeta);
+        }
+    }
+    double sc[S_NEG_RATIO];
+    HIPCHK(h, hipMemcpy(sc, h->s, sizeof(sc), hipMemcpyDeviceToHost));
+    v->ubarTop = sc[S_UBARTOP];
+    v->ubarTopp = sc[S_UBARTOPP];
+    v->ubarTopm = sc[S_UBARTOPM];
+    v->dubarTop_advect[0] = sc[S_DUBAR0];
+    v->dubarTop_advect[1] = sc[S_DUBAR1];
+    v->step = h->step;
+    v->status_for_adding_forcing = h->status_for_adding_forcing;
+    return VVM_OK;
+}
+
+// Host side of the random theta kick (src/Iteration.cpp:276-282): the
+// reference draws it on the host each step; so does this path, then copies it.
+static int upload_random_kick(vvm_dev *h) {
+    const size_t n = size_t(h->G.nx) * h->G.nz;
+    std::vector<double> host(n, 0.);
+    vvm_host_view v{};
+    v.init_th_forcing = host.data();
+    int rc = vvm_random_perturbation(&h->cfg, &v, h->step + 0, -0.001, 0.001, 1.);
+    if (rc) return fail(h, rc, "random perturbation failed");
+    std::vector<double> tmp;
+    to_dev_layout(h->G, host.data(), tmp);
+    HIPCHK(h, hipMemcpyAsync(h->forcing, tmp.data(), tmp.size() * sizeof(double), hipMemcpyHostToDevice, h->st));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return VVM_OK;
+}
+
+int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
+    if (!h || nsteps < 0) return VVM_ERR_ARG;
+    if (!h->uploaded) return fail(h, VVM_ERR_STATE, "step before upload");
+    HIPCHK(h, hipSetDevice(h->device));
+    const Grid &G = h->G;
+    const vvm_dev_config &c = h->cfg;
+    vvm_step_stats acc{};
+    int warn = 0;
+    const bool timing = (stats != nullptr);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int n = 0; n < nsteps; n++) {
+        h->step++;
+        h->status_for_adding_forcing = (h->step * c.dt <= c.addforcingtime) ? 1 : 0;
+        if (h->status_for_adding_forcing) {
+            int rc = upload_random_kick(h);
+            if (rc) return rc;
+        }
+        StepPtrs S = ptrs(h);
+        const bool last = (n == nsteps - 1) && timing;
+        if (last) hipEventRecord(h->ev[0], h->st);
+        // A: advection, buoyancy, forcing, halos (src/Iteration.cpp:263-285)
+        {
+            KTimer t(h, VVM_KT_PZETA);
+            launch_pzeta(G, S, h->step, h->st);
+        }
+        {
+            KTimer t(h, VVM_KT_THERMO4);
+            launch_thermo4(G, S, h->step, h->red.negval, h->st);
+        }
+        {
+            KTimer t(h, VVM_KT_SCALE);
+            launch_scale_forcing(G, S, true, c.tropical_forcing != 0, h->status_for_adding_forcing != 0, h->st);
+        }
+        if (h->ghosts_dirty) {
+            launch_bc_uw(G, S, h->st);
+            h->ghosts_dirty = false;
+        }
+        if (last) hipEventRecord(h->ev[1], h->st);
+        // B: pubarTop_pt, cal_w, cal_u (:292-296)
+        launch_pubartop(G, S, h->step, h->st);
+        vvm_solve_stats ss{};
+        int rc;
+        {
+            KTimer t(h, VVM_KT_SOLVE);
+            rc = solve_w(h, S, &ss);
+        }
+        if (rc == VVM_WARN_NOT_CONVERGED) {
+            warn = rc;
+            acc.not_converged++;
+        }
+        acc.solve_iters_total += ss.iters;
+        acc.solve_iters_last = ss.iters;
+        acc.refine_last = ss.refine;
+        acc.rel_residual_last = ss.rel_residual;
+        {
+            KTimer t(h, VVM_KT_CALU);
+            launch_cal_u(G, S, h->st);
+        }
+        if (last) hipEventRecord(h->ev[2], h->st);
+        // C: Kessler + NegativeValueProcess x3 + halos (:303-311, 404)
+        {
+            KTimer t(h, VVM_KT_KESSLER);
+            launch_kessler(G, S, h->red.negval, h->st);
+        }
+        {
+            KTimer t(h, VVM_KT_SCALE);
+            launch_scale_forcing(G, S, false, false, false, h->st);
+        }
+        if (last) hipEventRecord(h->ev[3], h->st);
+        // D: RKM_RKH with Mparam/Hparam, nudging (:408-417); the first
+        // updateMean (:408) is overwritten by nextTimeStep's before any read.
+        {
+            KTimer t(h, VVM_KT_TURB);
+            launch_turbulence_coeffs(G, S, h->st);
+        }
+        {
+            KTimer t(h, VVM_KT_DIFFUSION);
+            launch_diffusion(G, S, 1 | (c.CASE != 2 ? 2 : 0), h->st);
+        }
+        if (c.moisture_nudge_time != 0) launch_nudge_qv(G, S, c.moisture_nudge_time, h->st);
+        if (last) hipEventRecord(h->ev[4], h->st);
+        // E: nextTimeStep (:153-204)
+        {
+            KTimer t(h, VVM_KT_MEAN);
+            launch_update_mean(G, S, true, h->st);
+        }
+        rotate(h);
+        if (last) hipEventRecord(h->ev[5], h->st);
+        int lrc = check_launch(h);
+        if (lrc) return lrc;
+        h->stepped = true;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    kt_collect(h);
+    if (stats) {
+        *stats = acc;
+        stats->steps = nsteps;
+        if (nsteps > 0) {
+            float ms[5];
+            for (int e = 0; e < 5; e++) hipEventElapsedTime(&ms[e], h->ev[e], h->ev[e + 1]);
+            stats->t_advection = ms[0] * 1e-3;
+            stats->t_poisson = ms[1] * 1e-3;
+            stats->t_microphysics = ms[2] * 1e-3;
+            stats->t_diffusion = ms[3] * 1e-3;
+            stats->t_all = wall / nsteps;
+        }
+    }
+    return warn;
+}
+
+int vvm_dev_op(vvm_dev_t *h, int op) {
+    if (!h) return VVM_ERR_ARG;
+    if (!h->uploaded) return fail(h, VVM_ERR_STATE, "op before upload");
+    HIPCHK(h, hipSetDevice(h->device));
+    const Grid &G = h->G;
+    const vvm_dev_config &c = h->cfg;
+    StepPtrs S = ptrs(h);
+    int rc = VVM_OK;
+    if (op != VVM_OP_NEXT && h->p_alias_now) {
+        // the operator reads or writes the p arrays: materialise them
+        const size_t bytes = size_t(G.nz) * G.pitch * sizeof(double);
+        for (int f = 0; f < NPROG; f++)
+            HIPCHK(h, hipMemcpyAsync(S.fut[f], S.now[f], bytes, hipMemcpyDeviceToDevice, h->st));
+        h->p_alias_now = false;
+    }
+    switch (op) {
+        case VVM_OP_PZETA:
+            launch_pzeta(G, S, h->step, h->st);
+            break;
+        case VVM_OP_PSCALARS:
+            launch_thermo4(G, S, h->step, h->red.negval, h->st);
+            launch_scale_forcing(G, S, false, false, false, h->st);
+            break;
+        case VVM_OP_FORCING_BC: {
+            const double none[3] = {-1., -1., -1.};
+            HIPCHK(h, hipMemcpyAsync(h->s + S_NEG_RATIO, none, sizeof(none), hipMemcpyHostToDevice, h->st));
+            launch_scale_forcing(G, S, true, c.tropical_forcing != 0, h->status_for_adding_forcing != 0, h->st);
+            launch_bc_all_prog(G, S, h->st);
+            break;
+        }
+        case VVM_OP_PUBARTOP:
+            launch_pubartop(G, S, h->step, h->st);
+            break;
+        case VVM_OP_CAL_W: {
+            vvm_dev_config saved = h->cfg;
+            h->cfg.guess_mode = VVM_GUESS_ZERO;
+            rc = solve_w(h, S, nullptr);
+            h->cfg = saved;
+            break;
+        }
+        case VVM_OP_CAL_U:
+            launch_cal_u(G, S, h->st);
+            break;
+        case VVM_OP_MICROPHYSICS:
+            launch_kessler(G, S, h->red.negval, h->st);
+            launch_scale_forcing(G, S, false, false, false, h->st);
+            launch_bc_all_prog(G, S, h->st);
+            break;
+        case VVM_OP_TURBULENCE:
+            launch_turbulence_coeffs(G, S, h->st);
+            launch_diffusion(G, S, 0, h->st);
+            break;
+        case VVM_OP_NUDGE:
+            launch_nudge(G, S, 1 | (c.CASE != 2 ? 2 : 0), h->st);
+            if (c.moisture_nudge_time != 0) launch_nudge_qv(G, S, c.moisture_nudge_time, h->st);
+            break;
+        case VVM_OP_NEXT:
+            launch_update_mean(G, S, true, h->st);
+            rotate(h);
+            h->stepped = true;
+            break;
+        case VVM_OP_BC_ALL:
+            launch_bc_all_prog(G, S, h->st);
+            break;
+        default:
+            return fail(h, VVM_ERR_ARG, "unknown op");
+    }
+    int lrc = check_launch(h);
+    if (lrc) return lrc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    return rc;
+}
+
+static int poisson_common(vvm_dev *h, const double *d_b, double *d_x, vvm_solve_stats *stats) {
+    const vvm_dev_config &c = h->cfg;
+    h->pois.rhs_from_b(d_b, h->st);
+    h->pois.x_from_ref(d_x, h->st);
+    hipEventRecord(h->ev[6], h->st);
+    SolveResult R;
+    const int maxit = c.max_iter > 0 ? c.max_iter : 200;
+    if (c.solver_mode == VVM_SOLVER_MIXED) {
+        R = h->pois.solve_mixed(c.stop_mode == VVM_STOP_PARITY ? 1e-13 : c.tol_rel, 8, maxit, 1e-4, h->st);
+    } else {
+        R = h->pois.solve_fp64(c.stop_mode == VVM_STOP_PARITY ? 0. : c.tol_rel, maxit, h->st);
+    }
+    hipEventRecord(h->ev[7], h->st);
+    h->pois.x_to_ref(d_x, h->st);
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, h->ev[6], h->ev[7]);
+    if (stats) {
+        stats->iters = R.iters;
+        stats->refine = R.refine;
+        stats->rel_residual = R.rel;
+        stats->rel_residual0 = R.rel0;
+        stats->seconds = ms * 1e-3;
+    }
+    return R.converged ? VVM_OK : VVM_WARN_NOT_CONVERGED;
+}
+
+int vvm_dev_kernel_timer(vvm_dev_t *h, int kernel) {
+    if (!h || kernel < VVM_KT_NONE || kernel > VVM_KT_SOLVE) return VVM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    if (kernel != VVM_KT_NONE && h->kt_ev.empty()) {
+        h->kt_ev.resize(2 * vvm_dev::kKtMax);
+        for (auto &e : h->kt_ev) HIPCHK(h, hipEventCreate(&e));
+    }
+    h->kt_kernel = kernel;
+    h->kt_used = 0;
+    h->kt_ms = 0.;
+    h->kt_n = 0;
+    return VVM_OK;
+}
+
+int vvm_dev_kernel_time(vvm_dev_t *h, double *avg_seconds, int *launches) {
+    if (!h) return VVM_ERR_ARG;
+    if (avg_seconds) *avg_seconds = h->kt_n ? h->kt_ms * 1e-3 / double(h->kt_n) : 0.;
+    if (launches) *launches = int(h->kt_n);
+    return VVM_OK;
+}
+
+int vvm_dev_poisson_device(vvm_dev_t *h, const double *d_b, double *d_x, vvm_solve_stats *stats) {
+    if (!h || !d_b || !d_x) return VVM_ERR_ARG;
+    if (!h->pois_ready) return fail(h, VVM_ERR_STATE, "poisson before upload (needs rhou/rhow)");
+    HIPCHK(h, hipSetDevice(h->device));
+    return poisson_common(h, d_b, d_x, stats);
+}
+
+int vvm_dev_poisson(vvm_dev_t *h, const double *b, double *x, vvm_solve_stats *stats) {
+    if (!h || !b || !x) return VVM_ERR_ARG;
+    if (!h->pois_ready) return fail(h, VVM_ERR_STATE, "poisson before upload (needs rhou/rhow)");
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t n = size_t(h->G.nx - 2) * (h->G.nz - 3);
+    double *db = nullptr, *dx = nullptr;
+    HIPCHK(h, hipMalloc(&db, n * sizeof(double)));
+    if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess) {
+        hipFree(db);
+        return fail(h, VVM_ERR_NOMEM, "poisson: allocation failed");
+    }
+    hipMemcpy(db, b, n * sizeof(double), hipMemcpyHostToDevice);
+    hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice);
+    int rc = poisson_common(h, db, dx, stats);
+    if (rc >= 0) hipMemcpy(x, dx, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(db);
+    hipFree(dx);
+    return rc;
+}
+
+}  // extern "C"

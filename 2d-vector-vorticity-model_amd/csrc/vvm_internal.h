@@ -1,0 +1,69 @@
+// vvm_internal.h -- device state of one handle and the kernel launchers
+// shared between the translation units of libvvm_amd.so.
+#pragma once
+
+#include "dev_common.h"
+#include "vvm_dev.h"
+
+namespace vvm {
+
+// Field slots of the prognostic triplets (rotated instead of copied).
+enum { F_ZETA = 0, F_TH, F_QV, F_QC, F_QR, NPROG };
+// AB2 histories, two planes each ([slot] = reference's [i][k][slot]).
+enum { H_TH_ADV = 0, H_TH_BUOY, H_ZETA_ADV, H_QV_ADV, H_QC_ADV, H_QR_ADV, H_QR_VT, NHIST };
+
+struct Profiles {
+    double *rhou, *rhow, *pib, *pb, *thb, *thbm, *thb_init, *thb_zeta, *thvb, *thvbm;
+    double *qvb, *qvb0, *qvsb, *z, *z_zeta, *lambda2, *Q1LS, *Q2LS;
+    // per-level constants evaluated once on the host with the C library
+    double *vtfac;    // pow(rhow[k]/rhow[1], -0.5)            (Advection.cpp:165)
+    double *pcoef;    // 380 / (pow(pib[k], Cp/Rd) * P0)       (MicroPhysics.cpp:10,85)
+    double *rate_th;  // dt * CGR of Nudge_theta, 0 outside     (NumericalProcess.cpp:63-68)
+    double *rate_z;   // dt * CGR of Nudge_zeta, 0 outside      (NumericalProcess.cpp:77-82)
+};
+
+// Device scalars (one double each, in one small block).
+enum { S_UBARTOP = 0, S_UBARTOPP, S_UBARTOPM, S_DUBAR0, S_DUBAR1, S_NEG_RATIO, S_PRECIP_FAC = 9, NSCALAR = 16 };
+
+// Ratios of NegativeValueProcess (3 fields) live at s[S_NEG_RATIO + f]:
+// a value < 0 means "no rescale" (the reference's early return).
+
+// Pointers used by one timestep (the rotation resolved).
+struct StepPtrs {
+    double *now[NPROG], *past[NPROG], *fut[NPROG];
+    double *hist[NHIST][2];
+    double *u, *w, *RKM, *RKH, *forcing;
+    double *evap, *accr, *autoc, *cond;
+    double *precip, *xi, *uxi;
+    double *s;  // device scalars
+    Profiles P;
+};
+
+struct Reductions {
+    RedSlot negval;  // 6 sums: P,N for qv, qc, qr
+    RedSlot solver;  // solver dot products
+    RedSlot misc;
+};
+
+// ------------------------------------------------------------ launchers
+// Dynamics (k_dynamics.hip)
+void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st);
+void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
+                          hipStream_t st);
+void launch_bc_uw(const Grid &G, const StepPtrs &S, hipStream_t st);
+void launch_bc_all_prog(const Grid &G, const StepPtrs &S, hipStream_t st);
+void launch_pubartop(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+
+// Physics (k_physics.hip)
+void launch_kessler(const Grid &G, const StepPtrs &S, const RedSlot &rs, hipStream_t st);
+void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st);
+bool diffusion_supported(const Grid &G);  // nx - 2 <= 8192
+// nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta into the sweep's store.
+void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st);
+void launch_nudge(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st);
+void launch_nudge_qv(const Grid &G, const StepPtrs &S, double tau, hipStream_t st);
+void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipStream_t st);
+
+// Elliptic (k_poisson.hip): see poisson.h
+}  // namespace vvm

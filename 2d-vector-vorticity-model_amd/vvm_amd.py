@@ -23,11 +23,12 @@ LIB_PATH = os.path.join(PKG_DIR, "libvvm_amd.so")
 VVM_SOLVER_FP64, VVM_SOLVER_MIXED = 0, 1
 VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED = 0, 1, 2, 3
 VVM_STOP_PARITY, VVM_STOP_REL = 0, 1
+KT = {"none": 0, "pzeta": 1, "thermo4": 2, "scale": 3, "kessler": 4, "turb": 5, "diffusion": 6, "mean": 7,
+      "calu": 8, "solve": 9}
 
 OPS = {
-    "pzeta_pt": 1, "pth_pt": 2, "pqv_pt": 3, "pqc_pt": 4, "pqr_pt": 5, "forcing_bc": 6,
-    "pubarTop_pt": 7, "cal_w": 8, "cal_u": 9, "microphysics": 10, "turbulence": 11,
-    "nudge": 12, "nextTimeStep": 13, "bc_all": 14,
+    "pzeta_pt": 1, "pscalars": 2, "forcing_bc": 6, "pubarTop_pt": 7, "cal_w": 8, "cal_u": 9,
+    "microphysics": 10, "turbulence": 11, "nudge": 12, "nextTimeStep": 13, "bc_all": 14,
 }
 
 
@@ -103,6 +104,8 @@ def load_lib(path: str = LIB_PATH):
         "vvm_dev_poisson": (C.c_int, [C.c_void_p, PTR, PTR, P(VvmSolveStats)]),
         "vvm_dev_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, P(VvmSolveStats)]),
         "vvm_dev_op": (C.c_int, [C.c_void_p, C.c_int]),
+        "vvm_dev_kernel_timer": (C.c_int, [C.c_void_p, C.c_int]),
+        "vvm_dev_kernel_time": (C.c_int, [C.c_void_p, P(C.c_double), P(C.c_int)]),
         "vvm_init_case": (C.c_int, [P(VvmDevConfig), P(VvmCaseParams), P(VvmHostView)]),
         "vvm_random_perturbation": (C.c_int, [P(VvmDevConfig), P(VvmHostView), C.c_int,
                                               C.c_double, C.c_double, C.c_double]),
@@ -336,6 +339,14 @@ class Device:
         s = VvmStepStats()
         self._check(self.lib.vvm_dev_step(self.h, n, C.byref(s)), "step")
         return s
+
+    def kernel_timer(self, kernel: int) -> None:
+        self._check(self.lib.vvm_dev_kernel_timer(self.h, kernel), "kernel_timer")
+
+    def kernel_time(self) -> tuple[float, int]:
+        avg, n = C.c_double(), C.c_int()
+        self._check(self.lib.vvm_dev_kernel_time(self.h, C.byref(avg), C.byref(n)), "kernel_time")
+        return avg.value, n.value
 
     def op(self, name: str) -> int:
         return self._check(self.lib.vvm_dev_op(self.h, OPS[name]), f"op {name}")

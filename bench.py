@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X VVM timestep: grid-point-steps/s at 2048x256.
+
+One step = one pass of vvm::Iteration::TimeMarching's loop body (advection,
+buoyancy, forcing, elliptic solve for w and u, Kessler, turbulence, nudging)
+over the config-2 grid (2048 x 256 interior, dx = dz = 100 m, dt = 1 s,
+Gaussian theta noise, 100 W/m^2 surface heat flux, qv surface ramp), state
+resident in HBM.  The solve stops at ||b - A x|| / ||b|| <= 1e-10.
+
+Two paths are timed:
+  fp64_plain  -- FP64 MG-PCG from a zero guess (the reference's guess);
+  mixed       -- FP32 MG-PCG inside FP64 refinement from the best guess
+                 available (learned if trained weights exist, else the
+                 linear extrapolation of the last two solutions).
+`value` is the mixed path's throughput; both are printed.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path[:0] = [ROOT, os.path.join(ROOT, "2d-vector-vorticity-model_amd")]
+
+import vvm_amd as V  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per interior point and launch: distinct 2-D arrays
+# read + written (FP64), 1-D profiles and halo traffic excluded.
+KERNEL_BYTES_PER_POINT = {
+    "diffusion": 96.0,   # 5 fields read+write, RKM, RKH read
+    "thermo4": 160.0,    # 4 now + u, w + 5 old AB2 slots read; 4 fut + 5 new slots written
+    "pzeta": 96.0,       # zeta, u, w, th, qv, qc, qr + 2 old slots read; zetap + 2 new slots written
+    "kessler": 96.0,     # th, qv, qc, qr read and written; 4 process diagnostics written
+    "scale": 64.0,       # 4 fields read and written
+    "turb": 40.0,        # u, w, thp read; RKM, RKH written
+    "mean": 16.0,        # thp, qvp read
+    "calu": 24.0,        # w, zetap read; u written
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_path(case: V.Case, state: V.HostState, steps: int, warmup: int, kernel: str | None):
+    with V.Device(case.cfg) as dev:
+        dev.upload(state)
+        if warmup:
+            dev.step(warmup)
+        if kernel:
+            dev.kernel_timer(V.KT[kernel])
+        t0 = time.perf_counter()
+        stats = dev.step(steps)
+        wall = time.perf_counter() - t0
+        kavg, kn = dev.kernel_time() if kernel else (0.0, 0)
+        final = state.copy()
+        dev.download(final)
+    return wall, stats, kavg, kn, final
+
+
+def cpu_baseline(case: V.Case, state: V.HostState) -> dict | None:
+    """The reference compiled from its own sources (oracle/_ref), one thread,
+    on one step of the same workload; the C oracle if that binary is absent."""
+    import tempfile
+    from oracle import oracle as O
+    pts = (case.cfg.nx - 2) * (case.cfg.nz - 2)
+    if O.ref_available():
+        with tempfile.TemporaryDirectory() as tmp:
+            pin, pout = os.path.join(tmp, "in.state"), os.path.join(tmp, "out.state")
+            state.save(pin, case.cfg)
+            t0 = time.perf_counter()
+            O.run_ref("step", 1, pin, pout, timeout=900)
+            wall = time.perf_counter() - t0
+            r = V.HostState.read_records(pout)
+        t_all = float(r["t_all"][0])
+        return {"value": pts / t_all, "unit": "grid-point-steps/s", "cores": 1, "kind": "reference",
+                "sample": "1 step of the 2048x256 workload, reference model compiled from its sources "
+                          "(Eigen BiCGSTAB+ILUT, tol 1e-22), OMP_NUM_THREADS=1; value from its own t_all timer",
+                "t_poisson_s": float(r["t_poisson"][0]), "t_all_s": t_all, "process_wall_s": wall}
+    n = 3
+    st = state.copy()
+    t0 = time.perf_counter()
+    O.step(case.cfg, st, n)
+    wall = time.perf_counter() - t0
+    return {"value": pts * n / wall, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{n} steps of the 2048x256 workload, C restatement (direct FFT+Thomas solve), 1 thread"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--kernel", default="diffusion", choices=sorted(KERNEL_BYTES_PER_POINT))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--paths", default="fp64_plain,mixed")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the timestep is single-device; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    paths = {
+        "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
+        "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+    }
+    wanted = [p for p in args.paths.split(",") if p]
+    base = V.case("c2")
+    state = V.init_state(base)
+    pts = (base.cfg.nx - 2) * (base.cfg.nz - 2)
+    results, finals = {}, {}
+    for name in wanted:
+        c = V.case("c2", stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, **paths[name])
+        log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed steps of 2048x256")
+        wall, stats, kavg, kn, final = run_path(c, state, args.steps, args.warmup, args.kernel)
+        finals[name] = final
+        results[name] = {
+            "value": pts * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
+            "solve_iters_per_step": stats.solve_iters_total / max(args.steps, 1),
+            "refine_last": stats.refine_last, "rel_residual_last": stats.rel_residual_last,
+            "not_converged": stats.not_converged,
+            "phase_ms_last_step": {"advection": 1e3 * stats.t_advection, "poisson": 1e3 * stats.t_poisson,
+                                   "microphysics": 1e3 * stats.t_microphysics,
+                                   "diffusion": 1e3 * stats.t_diffusion},
+            "kernel_avg_us": 1e6 * kavg, "kernel_launches": kn,
+        }
+        log(f"[bench] {name}: {results[name]['value']:.4g} grid-point-steps/s, "
+            f"{results[name]['ms_per_step']:.3f} ms/step, {results[name]['solve_iters_per_step']:.2f} iters/step")
+
+    head = "mixed" if "mixed" in results else wanted[0]
+    R = results[head]
+    kbytes = KERNEL_BYTES_PER_POINT[args.kernel] * pts
+    achieved = kbytes / (R["kernel_avg_us"] * 1e-6) / 1e9 if R["kernel_avg_us"] > 0 else 0.0
+    out = {
+        "metric": "grid-point-steps/s at 2048x256, FP64 plain vs mixed+learned; Poisson iters to 1e-10",
+        "value": R["value"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "config 2: 2048x256 interior, dx=dz=100 m, dt=1 s, Kessler, sigma=0.1 K theta "
+                               "noise < 500 m, 100 W/m^2 surface heat flux, qv surface ramp; solve to 1e-10",
+                   "grid_points": pts, "path": head},
+        "paths": results,
+        "roofline": {"kernel": args.kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "bytes_per_launch": kbytes, "avg_launch_us": R["kernel_avg_us"], "traffic": None},
+    }
+    if "fp64_plain" in results and "mixed" in results:
+        out["speedup_mixed_vs_fp64"] = results["mixed"]["value"] / results["fp64_plain"]["value"]
+        a, b = finals["mixed"], finals["fp64_plain"]
+        out["mixed_vs_fp64_rel_l2"] = {f: float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300))
+                                       for f in ("zeta", "th", "qv", "u", "w")}
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline (one reference step at 2048x256, single thread)")
+        cb = cpu_baseline(base, state)
+        if cb:
+            out["cpu_baseline"] = cb
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -153,20 +153,37 @@ int vvm_dev_poisson_device(vvm_dev_t *h, const double *d_b, double *d_x, vvm_sol
 /* Per-operator parity hooks: run one reference operator on the device
  * state (tests).  op is one of VVM_OP_*. */
 #define VVM_OP_PZETA 1        /* Iteration::pzeta_pt   src/Iteration.cpp:29-34   */
-#define VVM_OP_PTH 2          /* Iteration::pth_pt     src/Iteration.cpp:36-40   */
-#define VVM_OP_PQV 3          /* Iteration::pqv_pt     src/Iteration.cpp:90-98   */
-#define VVM_OP_PQC 4          /* Iteration::pqc_pt     src/Iteration.cpp:100-107 */
-#define VVM_OP_PQR 5          /* Iteration::pqr_pt     src/Iteration.cpp:109-124 */
+/* pth_pt, pqv_pt, pqc_pt and pqr_pt (src/Iteration.cpp:36-40, 90-124) act
+ * on independent fields and run as one fused operator. */
+#define VVM_OP_PSCALARS 2
 #define VVM_OP_FORCING_BC 6   /* AddForcing + BoundaryProcess2D_all (:283-285)  */
 #define VVM_OP_PUBARTOP 7     /* PoissonSolver::pubarTop_pt :655-676            */
 #define VVM_OP_CAL_W 8        /* PoissonSolver::cal_w :439-477                  */
 #define VVM_OP_CAL_U 9        /* PoissonSolver::cal_u :580-653                  */
 #define VVM_OP_MICROPHYSICS 10/* Kessler + NegativeValueProcess x3 + BC :302-312,404 */
-#define VVM_OP_TURBULENCE 11  /* Turbulence::RKM_RKH (+Mparam/Hparam) Turbulent.cpp:5-66 */
+#define VVM_OP_TURBULENCE 11  /* Turbulence::RKM_RKH (+Mparam/Hparam) Turbulent.cpp:5-66, no nudging */
 #define VVM_OP_NUDGE 12       /* Nudge_theta/zeta/qv NumericalProcess.cpp:60-96 */
 #define VVM_OP_NEXT 13        /* Iteration::nextTimeStep :153-204               */
 #define VVM_OP_BC_ALL 14      /* BoundaryProcess2D_all Boundary.cpp:34-52       */
 int vvm_dev_op(vvm_dev_t *h, int op);
+
+/* Measurement aid (no counterpart in the reference, whose only
+ * instrumentation is the per-phase t_* timers): time every launch of one
+ * kernel of vvm_dev_step with HIP events on the handle's stream.
+ * vvm_dev_kernel_timer selects the kernel (VVM_KT_*, 0 = off) and resets the
+ * totals; vvm_dev_kernel_time returns the mean launch duration so far. */
+#define VVM_KT_NONE 0
+#define VVM_KT_PZETA 1       /* advection of zeta + buoyancy                 */
+#define VVM_KT_THERMO4 2     /* advection of th, qv, qc, qr + rain fall       */
+#define VVM_KT_SCALE 3       /* negative-value rescale + forcing + halos      */
+#define VVM_KT_KESSLER 4     /* Kessler microphysics                          */
+#define VVM_KT_TURB 5        /* RKM/RKH                                       */
+#define VVM_KT_DIFFUSION 6   /* in-place K-diffusion sweep (5 fields) + nudge */
+#define VVM_KT_MEAN 7        /* horizontal means (updateMean)                 */
+#define VVM_KT_CALU 8        /* cal_u (both kernels)                          */
+#define VVM_KT_SOLVE 9       /* the whole elliptic solve (many kernels)       */
+int vvm_dev_kernel_timer(vvm_dev_t *h, int kernel);
+int vvm_dev_kernel_time(vvm_dev_t *h, double *avg_seconds, int *launches);
 
 /* Host-side helpers (no GPU use). */
 

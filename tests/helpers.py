@@ -1,0 +1,89 @@
+"""Shared test helpers: seeded perturbed model states and field comparisons."""
+from __future__ import annotations
+
+import numpy as np
+
+import vvm_amd as V
+
+PROG = ["zeta", "th", "qv", "qc", "qr"]
+COMPARE_STEP = ["zeta", "zetam", "th", "thm", "qv", "qvm", "qc", "qcm", "qr", "qrm", "u", "w", "RKM", "RKH",
+                "dth_advect", "dth_buoyancy", "dzeta_advect", "dqv_advect", "dqc_advect", "dqr_advect", "dqr_VT",
+                "precip", "uxi", "thb", "qvb", "thvb", "evaporation", "accretion", "autoconversion",
+                "condensation"]
+
+
+def _bc_center(a):
+    a[0, 1:-1] = a[-2, 1:-1]
+    a[-1, 1:-1] = a[1, 1:-1]
+    a[:, 0] = a[:, 1]
+    a[:, -1] = a[:, -2]
+
+
+def _bc_westdown(a):
+    a[0, 1:-1] = a[-2, 1:-1]
+    a[-1, 1:-1] = a[1, 1:-1]
+    a[:, 0] = 0.
+    a[:, 1] = 0.
+    a[:, -1] = 0.
+
+
+def perturbed_state(case: V.Case, seed: int = 7, step: int = 3, consistent_ghosts: bool = True,
+                    negatives: bool = True) -> V.HostState:
+    """A state that exercises every branch of the step: non-zero winds,
+    cloud and rain (some negative), random AB2 histories and K fields."""
+    st = V.init_state(case)
+    rng = np.random.default_rng(seed)
+    nx, nz = st.nx, st.nz
+    sh = (nx, nz)
+    a = st.a
+    a["zeta"][:] = rng.normal(0, 2e-3, sh)
+    a["u"][:] = rng.normal(0, 3.0, sh)
+    a["w"][:] = rng.normal(0, 1.5, sh)
+    a["th"][:] += rng.normal(0, 0.5, sh)
+    a["qv"][:] *= 1 + rng.normal(0, 0.05, sh)
+    a["qc"][:] = rng.normal(6e-4, 8e-4 if negatives else 3e-4, sh)
+    # rain stays >= 0 (its fall speed is qr^0.1346); cloud may dip below 0
+    a["qr"][:] = np.abs(rng.normal(4e-4, 6e-4, sh))
+    a["RKM"][:] = rng.uniform(10, 300, sh)
+    a["RKH"][:] = rng.uniform(10, 300, sh)
+    for n in V.HISTS:
+        a[n][:] = rng.normal(0, 1e-5, (nx, nz, 2))
+    for f in PROG:
+        a[f + "m"][:] = a[f] + rng.normal(0, 1e-6, sh)
+        a[f + "p"][:] = rng.normal(0, 1, sh)  # overwritten by the step
+    if consistent_ghosts:
+        for f in ("th", "qv", "qc", "qr", "u", "RKM", "RKH"):
+            _bc_center(a[f])
+        _bc_westdown(a["zeta"])
+        _bc_westdown(a["w"])
+    st.s["ubarTop"] = 0.3
+    st.s["ubarTopp"] = 0.31
+    st.s["ubarTopm"] = 0.29
+    st.dubarTop_advect = [1e-4, -2e-4]
+    st.step = step
+    return st
+
+
+def rel_err(a: np.ndarray, b: np.ndarray) -> float:
+    nb = float(np.linalg.norm(b))
+    d = float(np.linalg.norm(np.asarray(a) - np.asarray(b)))
+    if nb == 0.0:
+        return 0.0 if d < 1e-300 else float("inf")
+    return d / nb
+
+
+def assert_fields_close(got: V.HostState, want: V.HostState, names, tol: float, where: str = ""):
+    bad = []
+    for n in names:
+        e = rel_err(got[n], want[n])
+        if not (e <= tol):
+            bad.append(f"{n}: rel {e:.3e}")
+    assert not bad, f"{where} fields above {tol:g}: " + ", ".join(bad)
+
+
+def gpu_run_op(dev: V.Device, st: V.HostState, op: str) -> V.HostState:
+    out = st.copy()
+    dev.upload(st)
+    dev.op(op)
+    dev.download(out)
+    return out

@@ -1,0 +1,131 @@
+"""Parity of the HIP timestep with the CPU oracle, through the C ABI.
+
+Tolerances (relative L2 over the whole array, ghosts included):
+  - single operators without a solve: 1e-12 (the GPU differs from the
+    oracle only where the device pow/exp/sqrt round differently);
+  - cal_w and anything downstream of the solve: 1e-10 (both sides solve
+    to round-off; the reference's own BiCGSTAB/ILUT result differs from the
+    oracle's direct solve by ~1e-14);
+  - one full step: 1e-10 per field, the bar the timestep must meet.
+"""
+import numpy as np
+import pytest
+
+import vvm_amd as V
+from helpers import COMPARE_STEP, assert_fields_close, gpu_run_op, perturbed_state, rel_err
+
+pytestmark = pytest.mark.gpu
+
+ALL2D = [n for n in V.FIELDS2D if n != "init_th_forcing"] + V.HISTS + ["precip", "uxi"]
+
+OPS = {
+    # GPU op -> (oracle ops in order, fields to compare, tolerance)
+    "pzeta_pt": (["pzeta_pt"], ["zetap", "dzeta_advect", "dth_buoyancy"], 1e-12),
+    "pscalars": (["pth_pt", "pqv_pt", "pqc_pt", "pqr_pt"],
+                 ["thp", "qvp", "qcp", "qrp", "dth_advect", "dqv_advect", "dqc_advect", "dqr_advect", "dqr_VT",
+                  "precip"], 1e-12),
+    "forcing_bc": (["forcing_bc"], ["thp", "qvp", "qcp", "qrp", "zetap", "u", "w"], 1e-13),
+    "cal_w": (["cal_w"], ["w"], 1e-10),
+    "cal_u": (["cal_u"], ["u", "uxi", "xi"], 1e-11),
+    "microphysics": (["microphysics"], ["thp", "qvp", "qcp", "qrp", "evaporation", "accretion", "autoconversion",
+                                        "condensation"], 1e-12),
+    "turbulence": (["turbulence"], ["RKM", "RKH", "zetap", "thp", "qvp", "qcp", "qrp"], 1e-12),
+    "nudge": (["nudge"], ["thp", "zetap", "qvp"], 1e-13),
+    "bc_all": (["bc_all"], ["zetap", "thp", "w", "u", "qvp", "qcp", "qrp"], 0.0),
+    "nextTimeStep": (["nextTimeStep"], ["zeta", "zetam", "zetap", "th", "thm", "qv", "qvm", "qc", "qcm", "qr",
+                                        "qrm", "thb", "qvb", "thvb", "thb_zeta"], 1e-14),
+}
+@pytest.fixture(scope="module")
+def tiny():
+    return V.case("tiny")
+
+
+@pytest.mark.parametrize("step", [3, 4])
+@pytest.mark.parametrize("op", list(OPS))
+def test_operator_parity(tiny, oracle, op, step):
+    st = perturbed_state(tiny, seed=11 + step, step=step)
+    if op in ("cal_u", "microphysics", "turbulence", "nudge", "nextTimeStep", "forcing_bc"):
+        # these act on the p-fields: give them the values a step would have
+        for f in ("zeta", "th", "qv", "qc", "qr"):
+            st[f + "p"][:] = st[f] + np.random.default_rng(step).normal(0, 1e-4, st[f].shape) * (
+                np.abs(st[f]).mean() + 1e-12)
+        st["qcp"][:] = np.abs(st["qcp"]) - 1e-5
+    oras, fields, tol = OPS[op]
+    want = st.copy()
+    for o in oras:
+        oracle.op(o, tiny.cfg, want)
+    with V.Device(tiny.cfg) as dev:
+        got = gpu_run_op(dev, st, op)
+    assert_fields_close(got, want, fields, tol, where=f"op {op} step {step}")
+
+
+def test_pubartop_parity(tiny, oracle):
+    st = perturbed_state(tiny, seed=5, step=6)
+    want = st.copy()
+    oracle.op("pubarTop_pt", tiny.cfg, want)
+    with V.Device(tiny.cfg) as dev:
+        got = gpu_run_op(dev, st, "pubarTop_pt")
+    assert abs(got.s["ubarTopp"] - want.s["ubarTopp"]) <= 1e-13 * max(1.0, abs(want.s["ubarTopp"]))
+    assert got.dubarTop_advect == pytest.approx(want.dubarTop_advect, rel=1e-12, abs=1e-18)
+
+
+@pytest.mark.parametrize("name,perturb", [("c0", False), ("c0", True), ("tiny", True)])
+def test_one_step_parity(oracle, name, perturb):
+    c = V.case(name)
+    st = perturbed_state(c, seed=3, step=0) if perturb else V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 1)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        stats = dev.step(1)
+        got = st.copy()
+        dev.download(got)
+    assert got.step == want.step == st.step + 1
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where=f"{name} 1 step")
+    assert stats.solve_iters_last > 0
+
+
+def test_one_step_parity_c1(oracle):
+    c = V.case("c1")
+    st = V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 1)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.step(1)
+        got = st.copy()
+        dev.download(got)
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="c1 1 step")
+
+
+def test_ten_steps_c0(oracle):
+    c = V.case("c0")
+    st = V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 10)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.step(10)
+        got = st.copy()
+        dev.download(got)
+    assert_fields_close(got, want, ["zeta", "th", "qv", "qc", "qr", "u", "w"], 1e-9, where="c0 10 steps")
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_poisson_isolated(oracle, mode):
+    c = V.case("c0", solver_mode=V.VVM_SOLVER_MIXED if mode == "mixed" else V.VVM_SOLVER_FP64,
+               stop_mode=V.VVM_STOP_REL, tol_rel=1e-10)
+    st = V.init_state(c)
+    nx, nz = c.cfg.nx, c.cfg.nz
+    n = (nx - 2) * (nz - 3)
+    b = np.random.default_rng(1).normal(0, 1, n)
+    xe = oracle.poisson_exact(c.cfg, st["rhou"], st["rhow"], b)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        x = np.zeros(n)
+        s = dev.poisson(b, x)
+    res = oracle.poisson_apply(c.cfg, st["rhou"], st["rhow"], x) - b
+    rr = np.linalg.norm(res) / np.linalg.norm(b)
+    assert rr <= 1e-10, rr
+    assert s.rel_residual <= 1e-10
+    assert rel_err(x, xe) <= 1e-6

@@ -1,0 +1,118 @@
+"""The oracle against the golden vectors of the compiled reference.
+
+tests/golden/*.npz were written by tests/golden/make_golden.py, which runs
+the reference model built from its own sources (oracle/ref/build_ref.sh).
+Stencil operators agree bit for bit (same evaluation order, no FMA, the same
+C library); operators that solve (cal_w's BiCGSTAB+ILUT against the oracle's
+direct solve, cal_u's CG against the closed form) agree to round-off.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import vvm_amd as V
+from helpers import rel_err
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ORACLE_OP = {"pzeta_pt": "pzeta_pt", "pth_pt": "pth_pt", "pqv_pt": "pqv_pt", "pqc_pt": "pqc_pt",
+             "pqr_pt": "pqr_pt", "forcing_bc": "forcing_bc", "bc_all": "bc_all", "pubarTop_pt": "pubarTop_pt",
+             "cal_w": "cal_w", "cal_u": "cal_u", "microphysics": "microphysics", "turbulence": "turbulence",
+             "nudge": "nudge", "updateMean": "updateMean", "nextTimeStep": "nextTimeStep"}
+# Operators whose result comes out of an iterative solve in the reference.
+SOLVE_TOL = {"cal_w": 1e-12, "cal_u": 1e-12}
+SCAL = {"ubarTop", "ubarTopp", "ubarTopm", "dubarTop_advect0", "dubarTop_advect1", "step"}
+
+
+def load_state(d, prefix, like: V.HostState) -> V.HostState:
+    st = like.copy()
+    for n in st.a:
+        st.a[n][...] = d[prefix + n].reshape(st.a[n].shape)
+    for n in ("ubarTop", "ubarTopp", "ubarTopm"):
+        st.s[n] = float(d[prefix + n][0])
+    st.dubarTop_advect = [float(d[prefix + "dubarTop_advect0"][0]), float(d[prefix + "dubarTop_advect1"][0])]
+    st.step = int(d[prefix + "step"][0])
+    return st
+
+
+@pytest.fixture(scope="module")
+def ops_gold():
+    return np.load(os.path.join(GOLD, "ops_tiny.npz"))
+
+
+@pytest.mark.parametrize("step", [3, 4])
+@pytest.mark.parametrize("op", list(ORACLE_OP))
+def test_operator_vs_reference(oracle, ops_gold, op, step):
+    c = V.case("tiny")
+    like = V.HostState(c.cfg.nx, c.cfg.nz)
+    st = load_state(ops_gold, f"s{step}/in/", like)
+    oracle.op(ORACLE_OP[op], c.cfg, st, gsolve=oracle.GSOLVE_EXACT)
+    keys = [k for k in ops_gold.files if k.startswith(f"s{step}/{op}/")]
+    assert keys, "fixture lists no changed arrays"
+    tol = SOLVE_TOL.get(op, 0.0)
+    for k in keys:
+        name = k.split("/")[-1]
+        want = ops_gold[k]
+        if name in SCAL:
+            got = {"ubarTop": st.s["ubarTop"], "ubarTopp": st.s["ubarTopp"], "ubarTopm": st.s["ubarTopm"],
+                   "dubarTop_advect0": st.dubarTop_advect[0], "dubarTop_advect1": st.dubarTop_advect[1],
+                   "step": st.step}[name]
+            assert got == pytest.approx(float(want[0]), rel=max(tol, 1e-15), abs=1e-300), (op, name)
+            continue
+        if name in ("xi",):  # the 1-D potential is defined up to the solver's round-off
+            assert rel_err(st[name], want.reshape(st[name].shape)) <= 1e-10, (op, name)
+            continue
+        got = st[name]
+        e = rel_err(got, want.reshape(got.shape))
+        assert e <= tol, f"{op}/{name}: rel {e:.3e} > {tol}"
+
+
+def test_one_and_ten_steps_vs_reference(oracle):
+    d = np.load(os.path.join(GOLD, "steps.npz"))
+    c = V.case("c0")
+    st = V.init_state(c)
+    o = st.copy()
+    oracle.step(c.cfg, o, 1)
+    for f in ("zeta", "th", "qv", "qc", "qr", "RKM", "RKH", "thb", "qvb", "thvb", "precip"):
+        assert rel_err(o[f], d[f"c0/1/{f}"].reshape(o[f].shape)) == 0.0, f
+    for f in ("u", "w", "uxi"):
+        assert rel_err(o[f], d[f"c0/1/{f}"].reshape(o[f].shape)) <= 1e-12, f
+    oracle.step(c.cfg, o, 9)
+    for f in ("zeta", "th", "qv", "qc", "qr", "u", "w", "RKM", "RKH"):
+        assert rel_err(o[f], d[f"c0/10/{f}"].reshape(o[f].shape)) <= 1e-11, f
+
+
+def test_twenty_steps_perturbed_vs_reference(oracle):
+    d = np.load(os.path.join(GOLD, "steps.npz"))
+    c = V.case("tiny")
+    st = load_state(d, "tiny/in/", V.HostState(c.cfg.nx, c.cfg.nz))
+    oracle.step(c.cfg, st, 20)
+    for f in ("zeta", "th", "qv", "qc", "qr", "u", "w", "RKM", "RKH"):
+        assert rel_err(st[f], d[f"tiny/20/{f}"].reshape(st[f].shape)) <= 1e-10, f
+
+
+def test_cal_w_vs_reference_solve(oracle):
+    d = np.load(os.path.join(GOLD, "misc.npz"))
+    c = V.case("c0")
+    st = V.init_state(c)
+    st["zetap"][:] = d["calw/zetap"]
+    oracle.op("cal_w", c.cfg, st)
+    assert rel_err(st["w"], d["calw/w"]) <= 1e-12
+    # the reference at tol 1e-10 (interior only: the harness's solve mode
+    # skips the halo call) is within its residual of the exact solution
+    assert rel_err(st["w"][1:-1, 2:-1], d["calw/w_1e-10"][1:-1, 2:-1]) <= 1e-7
+    assert float(d["calw/relres_1e-10"][0]) <= 1e-10
+
+
+def test_random_perturbation_matches_reference():
+    """Init::RandomPerturbation: the product's host generator reproduces the
+    reference's mt19937 + normal_distribution stream exactly."""
+    d = np.load(os.path.join(GOLD, "misc.npz"))
+    lib = V.load_lib()
+    import ctypes as C
+    c = V.grid_case(24, 16, dx=50.0, dt=1.0)
+    for t in (1, 7):
+        st = V.HostState(c.cfg.nx, c.cfg.nz)
+        v = st.view()
+        assert lib.vvm_random_perturbation(C.byref(c.cfg), C.byref(v), t, -0.001, 0.001, 1.0) == 0
+        assert np.array_equal(st["init_th_forcing"], d[f"rp/{t}"].reshape(st["init_th_forcing"].shape))
