@@ -11,7 +11,7 @@
 #include <cstddef>
 #include <cstdint>
 
-namespace vvm {
+namespace vvmd {
 
 struct Grid {
     int nx, nz, pitch;
@@ -87,51 +87,89 @@ __device__ inline void block_sum(T (&v)[NV]) {
     __syncthreads();
 }
 
-// Scratch of one grid-wide reduction: per-block partials and an arrival
-// counter that the last block resets to 0.
+// Grid-wide FP64 sums are two launches: every block of the producing kernel
+// writes its block sums with plain stores (block_partials), then one
+// workgroup (k_finalize) adds them in block order and derives the scalars
+// the next kernels read.  The kernel boundary orders the two; no fence or
+// atomic is needed inside the producer.  (A last-block-done pattern needs an
+// agent-scope release per block, which on a GPU with one L2 per XCD costs an
+// L2 write-back each time.)  The order is fixed, so results are
+// deterministic from run to run.
 struct RedSlot {
-    double *partial;    // [NV][max_blocks]
-    unsigned *counter;  // one word
+    double *partial;  // [8][max_blocks]
     int max_blocks;
 };
 
-// Grid-wide FP64 sums in a fixed order (deterministic): every block writes
-// its partials; the block that arrives last sums all partials in block order
-// and gets `true` with the totals in tot[] (in all its threads).  Every
-// other block gets `false`.
 template <int NV>
-__device__ inline bool grid_sum_last(double (&v)[NV], const RedSlot &rs, double (&tot)[NV]) {
-    __shared__ bool is_last;
+__device__ inline void block_partials(double (&v)[NV], const RedSlot &rs) {
     const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
-    const int nthr = blockDim.x * blockDim.y * blockDim.z;
-    const int nblk = gridDim.x * gridDim.y * gridDim.z;
     const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     block_sum<NV>(v);
     if (tid == 0) {
 #pragma unroll
         for (int j = 0; j < NV; j++) rs.partial[size_t(j) * rs.max_blocks + bid] = v[j];
-        __threadfence();
-        unsigned prev = atomicAdd(rs.counter, 1u);
-        is_last = (prev == unsigned(nblk - 1));
     }
-    __syncthreads();
-    if (!is_last) return false;
-    __threadfence();
+}
+
+// Scalars of the preconditioned CG (doubles in its scalar block).
+enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, NPSCAL = 8 };
+
+// What k_finalize does with the totals.
+enum RedMode {
+    RED_STORE = 0,   // sc[dst+j] = tot[j] (and sc[dst2+j] when dst2 >= 0)
+    RED_NEGVAL,      // NegativeValueProcess ratios of 3 fields into sc[dst+q]
+    RED_ALPHA,       // CG: sc[P_PQ] = p.q, sc[P_ALPHA] = sc[P_RZ] / p.q
+    RED_BETA,        // CG: sc[P_BETA] = rz / sc[P_RZ], sc[P_RZ] = rz
+    RED_RZ_INIT      // CG: sc[P_RZ] = rz
+};
+
+namespace {
+template <int NV>
+__global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *sc, int mode, int dst, int dst2) {
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0.;
-    // Fixed-order partial sums: thread t sums blocks t, t+nthr, ...
-    for (int b = tid; b < nblk; b += nthr) {
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
 #pragma unroll
-        for (int j = 0; j < NV; j++)
-            acc[j] += __hip_atomic_load(rs.partial + size_t(j) * rs.max_blocks + b, __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT);
+        for (int j = 0; j < NV; j++) acc[j] += rs.partial[size_t(j) * rs.max_blocks + b];
     }
     block_sum<NV>(acc);
-#pragma unroll
-    for (int j = 0; j < NV; j++) tot[j] = acc[j];
-    if (tid == 0) *rs.counter = 0u;
-    return true;
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+        case RED_STORE:
+            for (int j = 0; j < NV; j++) {
+                sc[dst + j] = acc[j];
+                if (dst2 >= 0) sc[dst2 + j] = acc[j];
+            }
+            break;
+        case RED_NEGVAL:
+            // MicroPhysics.cpp:129-131: no rescale when nothing is positive or
+            // the negatives outweigh the positives (marked by a ratio < 0).
+            for (int q = 0; q < NV / 2; q++) {
+                const double P = acc[2 * q], N = acc[2 * q + 1];
+                sc[dst + q] = (P == 0. || fabs(N) > P) ? -1. : 1. - fabs(N / P);
+            }
+            break;
+        case RED_ALPHA:
+            sc[P_PQ] = acc[0];
+            sc[P_ALPHA] = sc[P_RZ] / acc[0];
+            break;
+        case RED_BETA:
+            sc[P_BETA] = acc[0] / sc[P_RZ];
+            sc[P_RZ] = acc[0];
+            break;
+        case RED_RZ_INIT:
+            sc[P_RZ] = acc[0];
+            break;
+    }
+}
+}  // namespace
+
+template <int NV>
+inline void finalize(const RedSlot &rs, int nblk, double *sc, int mode, int dst, hipStream_t st, int dst2 = -1) {
+    hipLaunchKernelGGL(k_finalize<NV>, dim3(1), dim3(256), 0, st, rs, nblk, sc, mode, dst, dst2);
 }
 
-}  // namespace vvm
+inline int nblocks(dim3 g) { return int(g.x * g.y * g.z); }
+
+}  // namespace vvmd

@@ -1,0 +1,229 @@
+// k_diffusion.hip -- Turbulence::Mparam / Hparam (src/Turbulent.cpp:68-114)
+// on the device, with Nudge_theta / Nudge_zeta fused into the store.
+//
+// The reference updates each field in place, k outer and i inner, so a new
+// value depends on the *new* west and lower neighbours:
+//   new(i,k) = a(i,k) new(i-1,k) + c(i,k) + d(i,k) new(i,k-1),
+// where a, d hold the diffusion coefficients and c the old centre, east and
+// upper values; ghost cells are never updated during the sweep.
+//
+// Two kernels.  k_diff_coeffs forms a, c, d for every point of the five
+// fields in parallel over the whole GPU (a and d of Hparam depend only on
+// RKH and are shared by th, qv, qc, qr).  k_diff_sweep then does only the
+// sequential part: row k is a first-order linear recurrence in i once row
+// k-1 is final.  One workgroup per field; wavefront w owns a strip of 64*E
+// consecutive columns and walks the rows in order.  In each row a wave
+// composes its lanes' affine maps y -> a y + e (e = c + d y_below) with a
+// 64-lane scan, takes the carry (the new value at the last column of strip
+// w-1) from LDS, applies it and publishes its own last value.  The waves
+// form a pipeline along the rows.
+//
+// The carry slots start as a NaN sentinel that arithmetic never produces; a
+// wave spins (relaxed LDS loads, no fence) until its slot holds a value.  A
+// wave waits only for the wave on its left and wave 0 never waits, so every
+// wait ends; a spin limit turns a wait that does not end into an error flag
+// instead of a hang.  Values agree with the sequential sweep to rounding.
+#pragma clang fp contract(off)
+
+#include "vvm_internal.h"
+
+namespace vvmd {
+
+namespace {
+
+constexpr int kMaxWaves = 16;
+constexpr int kSpinLimit = 1 << 24;
+constexpr unsigned long long kEmpty = 0x7FF4DEADBEEF0001ull;  // a NaN payload arithmetic never makes
+constexpr int BX = 64, BY = 4;
+
+struct Affine {
+    double a, b;  // y = a x + b
+};
+
+// g after f: y = g.a (f.a x + f.b) + g.b
+__device__ inline Affine after(const Affine &g, const Affine &f) { return {g.a * f.a, g.a * f.b + g.b}; }
+
+// a, c, d of every interior point (Turbulent.cpp:81-86 for Mparam on zeta,
+// :105-110 for Hparam on th, qv, qc, qr).  Row k = 1 folds the (old) ghost
+// row below into c.
+__global__ void __launch_bounds__(BX *BY) k_diff_coeffs(Grid G, StepPtrs S) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const size_t p = ix(G, i, k);
+    const double A = G.rdx2 * G.dt;
+    const double *rhou = S.P.rhou, *rhow = S.P.rhow;
+    {  // Mparam
+        const double *K = S.RKM, *f = S.fut[F_ZETA];
+        const double Ke = 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
+        const double Kw = 0.5 * (K[ix(G, i - 1, k)] + K[ix(G, i - 1, k - 1)]);
+        const double Ct = rhou[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
+        const double Cb = rhou[k - 1] * 0.5 * (K[ix(G, i, k - 1)] + K[ix(G, i - 1, k - 1)]);
+        const double Bk = G.rdz2 * G.dt / rhow[k];
+        const double a = A * Kw, d = Bk * Cb;
+        const double old = f[p], oldE = f[ix(G, i + 1, k)], oldT = f[ix(G, i, k + 1)];
+        double c = old + A * (Ke * (oldE - old) - Kw * old) + Bk * (Ct * (oldT - old) - Cb * old);
+        if (k == 1) c = c + d * f[ix(G, i, 0)];
+        S.dcoef[DC_AM][p] = a;
+        S.dcoef[DC_DM][p] = (k == 1) ? 0. : d;
+        S.dcoef[DC_C0 + F_ZETA][p] = c;
+    }
+    {  // Hparam
+        const double *K = S.RKH;
+        const double Ke = 0.5 * (K[ix(G, i + 1, k)] + K[ix(G, i, k)]);
+        const double Kw = 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
+        const double Ct = rhow[k + 1] * 0.5 * (K[ix(G, i, k + 1)] + K[ix(G, i, k)]);
+        const double Cb = rhow[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
+        const double Bk = G.rdz2 * G.dt / rhou[k];
+        const double a = A * Kw, d = Bk * Cb;
+        S.dcoef[DC_AH][p] = a;
+        S.dcoef[DC_DH][p] = (k == 1) ? 0. : d;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int fld = F_TH + q;
+            const double *f = S.fut[fld];
+            const double old = f[p], oldE = f[ix(G, i + 1, k)], oldT = f[ix(G, i, k + 1)];
+            double c = old + A * (Ke * (oldE - old) - Kw * old) + Bk * (Ct * (oldT - old) - Cb * old);
+            if (k == 1) c = c + d * f[ix(G, i, 0)];
+            S.dcoef[DC_C0 + fld][p] = c;
+        }
+    }
+}
+
+// Rows of coefficients kept in flight per lane: the row-to-row chain is far
+// shorter than a load from memory, so loads are issued PF rows ahead into a
+// circular register buffer (the row loop is unrolled by PF so the buffer is
+// statically indexed).
+constexpr int PF = 4;
+
+template <int E>
+__device__ inline void load_row(const Grid &G, const double *__restrict__ A_, const double *__restrict__ C_,
+                                const double *__restrict__ D_, int i0, int n, int k, int nz, double (&a)[E],
+                                double (&c)[E], double (&d)[E]) {
+#pragma unroll
+    for (int j = 0; j < E; j++) {
+        a[j] = c[j] = d[j] = 0.;
+        if (k <= nz - 2 && i0 + j <= n) {
+            const size_t p = ix(G, i0 + j, k);
+            a[j] = A_[p];
+            c[j] = C_[p];
+            d[j] = D_[p];
+        }
+    }
+}
+
+template <int E>
+__device__ void sweep(const Grid &G, double *__restrict__ f, const double *__restrict__ A_,
+                      const double *__restrict__ C_, const double *__restrict__ D_, const double *rate, int is_theta,
+                      const double *thb_init, unsigned long long *carry, double *err) {
+    const int n = G.nx - 2, nz = G.nz;
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    const int strip0 = 1 + w * kWave * E;               // first column of the strip
+    const int strip1 = min(strip0 + kWave * E - 1, n);  // last column of the strip
+    const int i0 = strip0 + lane * E;
+
+    double ba[PF][E], bc[PF][E], bd[PF][E], ynew[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) ynew[j] = 0.;
+#pragma unroll
+    for (int q = 0; q < PF; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
+    for (int k0 = 1; k0 <= nz - 2; k0 += PF) {
+#pragma unroll
+    for (int q = 0; q < PF; q++) {
+        const int k = k0 + q;
+        if (k > nz - 2) break;
+        double a[E], e[E];
+        Affine m{1., 0.};
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            a[j] = ba[q][j];
+            e[j] = bc[q][j] + bd[q][j] * ynew[j];
+            if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
+        }
+        // refill this slot with row k + PF
+        load_row<E>(G, A_, C_, D_, i0, n, k + PF, nz, ba[q], bc[q], bd[q]);
+        // inclusive scan over the wave, then the exclusive prefix of this lane
+        Affine inc = m;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double pa = __shfl_up(inc.a, off, kWave), pb = __shfl_up(inc.b, off, kWave);
+            if (lane >= off) inc = after(inc, Affine{pa, pb});
+        }
+        const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
+        // carry: the new value left of the strip
+        double y0;
+        if (w == 0) {
+            y0 = f[ix(G, 0, k)];  // ghost column, old as in the reference
+        } else {
+            unsigned long long bits = kEmpty;
+            if (lane == 0) {
+                int spins = 0;
+                unsigned long long *slot = carry + (w - 1) * nz + k;
+                while ((bits = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) == kEmpty) {
+                    if (++spins > kSpinLimit) {
+                        *err = 1.;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            y0 = __shfl(__longlong_as_double(static_cast<long long>(bits)), 0, kWave);
+        }
+        double y = (lane == 0) ? y0 : ea * y0 + eb;
+        const double r = rate ? rate[k] : 0.;
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int i = i0 + j;
+            if (i > n) continue;
+            y = a[j] * y + e[j];
+            ynew[j] = y;
+            double out = y;
+            if (r != 0.) out = is_theta ? y - r * (y - thb_init[k]) : y - r * y;
+            f[ix(G, i, k)] = out;
+            if (i == strip1)
+                __hip_atomic_store(carry + w * nz + k, static_cast<unsigned long long>(__double_as_longlong(y)),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+    }
+    }
+}
+
+// blockIdx.x: 0 zeta (Mparam), 1..4 th, qv, qc, qr (Hparam).
+// nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta.
+template <int E>
+__global__ void __launch_bounds__(kMaxWaves *kWave) k_diff_sweep(Grid G, StepPtrs S, int nudge) {
+    extern __shared__ unsigned long long carry[];  // [waves][nz]
+    const int waves = blockDim.x / kWave;
+    for (int t = threadIdx.x; t < waves * G.nz; t += blockDim.x) carry[t] = kEmpty;
+    __syncthreads();
+    const int fld = blockIdx.x;  // F_ZETA .. F_QR
+    const bool mpar = (fld == F_ZETA);
+    const double *rate = mpar ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
+    sweep<E>(G, S.fut[fld], S.dcoef[mpar ? DC_AM : DC_AH], S.dcoef[DC_C0 + fld], S.dcoef[mpar ? DC_DM : DC_DH], rate,
+             fld == F_TH, S.P.thb_init, carry, S.s + S_DIFF_ERR);
+}
+
+template <int E>
+void launch_e(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+    const int n = G.nx - 2;
+    const int waves = (n + kWave * E - 1) / (kWave * E);
+    const size_t lds = size_t(waves) * G.nz * sizeof(unsigned long long);
+    hipLaunchKernelGGL(k_diff_sweep<E>, dim3(NPROG), dim3(waves * kWave), lds, st, G, S, nudge);
+}
+
+}  // namespace
+
+// nx - 2 <= 16 waves * 64 lanes * 8 columns, and the carry rows fit in LDS.
+bool diffusion_supported(const Grid &G) {
+    return G.nx - 2 <= kMaxWaves * kWave * 8 && size_t(kMaxWaves) * G.nz * sizeof(double) <= 60 * 1024;
+}
+
+void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+    hipLaunchKernelGGL(k_diff_coeffs, dim3((G.nx + BX - 1) / BX, (G.nz + BY - 1) / BY), dim3(BX, BY), 0, st, G, S);
+    const int n = G.nx - 2;
+    if (n <= kMaxWaves * kWave) launch_e<1>(G, S, nudge, st);
+    else if (n <= 2 * kMaxWaves * kWave) launch_e<2>(G, S, nudge, st);
+    else if (n <= 4 * kMaxWaves * kWave) launch_e<4>(G, S, nudge, st);
+    else launch_e<8>(G, S, nudge, st);
+}
+
+}  // namespace vvmd

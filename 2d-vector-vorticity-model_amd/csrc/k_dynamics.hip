@@ -11,7 +11,7 @@
 
 #include "vvm_internal.h"
 
-namespace vvm {
+namespace vvmd {
 
 namespace {
 
@@ -179,8 +179,8 @@ __device__ inline double vt_flux(const Grid &G, const double *qr, const double *
 // pth_pt, pqv_pt, pqc_pt, pqr_pt (src/Iteration.cpp:36-40, 90-124): the four
 // scalar advections sharing u and w, rain sedimentation on qr, surface
 // precipitation, and the first pass of NegativeValueProcess (zero the
-// negatives, sum positives and negatives) for qv, qc, qr.  The last block
-// turns the sums into rescale ratios.
+// negatives, sum positives and negatives) for qv, qc, qr.  k_finalize turns
+// the sums into rescale ratios.
 __global__ void __launch_bounds__(BX *BY) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
     const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
     const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
@@ -225,16 +225,7 @@ __global__ void __launch_bounds__(BX *BY) k_thermo4(Grid G, StepPtrs S, int step
             S.precip[i] = (rain < 0) ? 0. : rain;
         }
     }
-    double tot[6];
-    if (grid_sum_last<6>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            for (int q = 0; q < 3; q++) {
-                const double P = tot[2 * q], N = tot[2 * q + 1];
-                const bool skip = (P == 0. || fabs(N) > P);
-                S.s[S_NEG_RATIO + q] = skip ? -1. : 1. - fabs(N / P);
-            }
-        }
-    }
+    block_partials<6>(part, rs);
 }
 
 // Second pass of NegativeValueProcess for qv, qc, qr (MicroPhysics.cpp:
@@ -339,6 +330,7 @@ void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
 }
 void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st) {
     hipLaunchKernelGGL(k_thermo4, grid2d(G), dim3(BX, BY), 0, st, G, S, step, rs);
+    finalize<6>(rs, nblocks(grid2d(G)), S.s, RED_NEGVAL, S_NEG_RATIO, st);
 }
 void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
                           hipStream_t st) {
@@ -355,4 +347,4 @@ void launch_pubartop(const Grid &G, const StepPtrs &S, int step, hipStream_t st)
     hipLaunchKernelGGL(k_pubartop, dim3(1), dim3(1024), 0, st, G, S, step);
 }
 
-}  // namespace vvm
+}  // namespace vvmd

@@ -9,7 +9,7 @@
 
 #include "vvm_internal.h"
 
-namespace vvm {
+namespace vvmd {
 
 namespace {
 
@@ -24,7 +24,7 @@ inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY
 // (MicroPhysics.cpp:29-51, 54-77, 80-114, 7-26).  Each is pointwise, so
 // running the four in call order per point equals the reference's four
 // sweeps.  Then the first pass of NegativeValueProcess on qvp, qcp, qrp
-// (src/Iteration.cpp:309-311); the last block computes the three ratios.
+// (src/Iteration.cpp:309-311); k_finalize computes the three ratios.
 __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot rs) {
     const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
     const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
@@ -88,15 +88,7 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
         S.fut[F_QR][p] = q3[2];
         S.fut[F_TH][p] = th;
     }
-    double tot[6];
-    if (grid_sum_last<6>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            for (int q = 0; q < 3; q++) {
-                const double P = tot[2 * q], N = tot[2 * q + 1];
-                S.s[S_NEG_RATIO + q] = (P == 0. || fabs(N) > P) ? -1. : 1. - fabs(N / P);
-            }
-        }
-    }
+    block_partials<6>(part, rs);
 }
 
 // Turbulence::RKM_RKH coefficient part (Turbulent.cpp:13-45), with the
@@ -131,118 +123,6 @@ __global__ void __launch_bounds__(BX *BY) k_turb_coeffs(Grid G, StepPtrs S) {
     kh = mins(kh, kmax);
     store_center(S.RKM, G, i, k, km);
     store_center(S.RKH, G, i, k, kh);
-}
-
-// ------------------------------------------------------------ diffusion
-//
-// Turbulence::Mparam / Hparam (Turbulent.cpp:68-114) update a field in
-// place, k outer and i inner, so each new value depends on the *new* west
-// and lower neighbours: new(i,k) = a(i,k) new(i-1,k) + e(i,k), where e holds
-// the old centre/east/upper values and the new lower value.  Ghost cells are
-// never updated during the sweep.  Row k is therefore a first-order linear
-// recurrence in i once row k-1 is final: one workgroup per field walks the
-// rows in order and solves each row with a parallel scan of affine maps
-// y -> a y + e.  The values agree with the sequential sweep to rounding.
-// The Rayleigh nudging of theta and zeta (NumericalProcess.cpp:60-84),
-// applied after the sweep in the reference, is fused into the store: the
-// recurrence keeps the un-nudged values.
-
-constexpr int DT = 1024;  // threads per field workgroup
-
-struct Affine {
-    double a, b;  // y = a x + b
-};
-
-// g after f: y = g.a (f.a x + f.b) + g.b
-__device__ inline Affine after(const Affine &g, const Affine &f) { return {g.a * f.a, g.a * f.b + g.b}; }
-
-// Exclusive scan over the block; returns the composition of the maps of all
-// lower threads (identity for thread 0).
-__device__ inline Affine block_exclusive_scan(Affine m) {
-    __shared__ double sa[DT / kWave], sb[DT / kWave];
-    const int tid = threadIdx.x, lane = tid % kWave, wid = tid / kWave;
-    Affine inc = m;
-#pragma unroll
-    for (int off = 1; off < kWave; off <<= 1) {
-        const double pa = __shfl_up(inc.a, off, kWave), pb = __shfl_up(inc.b, off, kWave);
-        if (lane >= off) inc = after(inc, Affine{pa, pb});
-    }
-    if (lane == kWave - 1) {
-        sa[wid] = inc.a;
-        sb[wid] = inc.b;
-    }
-    __syncthreads();
-    Affine wpre{1., 0.};
-    for (int w2 = 0; w2 < wid; w2++) wpre = after(Affine{sa[w2], sb[w2]}, wpre);
-    const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
-    Affine ex = (lane == 0) ? wpre : after(Affine{ea, eb}, wpre);
-    __syncthreads();
-    return ex;
-}
-
-// E consecutive columns per thread (compile-time so the row state stays in
-// registers); E * DT >= nx - 2.
-template <int E>
-__global__ void __launch_bounds__(DT) k_diffusion(Grid G, StepPtrs S, int nudge) {
-    const int fld = blockIdx.x;  // 0: zeta (Mparam), 1..4: th, qv, qc, qr (Hparam)
-    const int fidx[5] = {F_ZETA, F_TH, F_QV, F_QC, F_QR};
-    double *f = S.fut[fidx[fld]];
-    const bool mpar = (fld == 0);
-    const double *K = mpar ? S.RKM : S.RKH;
-    const double *rhou = S.P.rhou, *rhow = S.P.rhow;
-    // nudge bit 0: Nudge_theta, bit 1: Nudge_zeta
-    const double *rate = (fld == 0) ? ((nudge & 2) ? S.P.rate_z : nullptr)
-                                    : ((fld == 1 && (nudge & 1)) ? S.P.rate_th : nullptr);
-    const double *thb_init = S.P.thb_init;
-    const int n = G.nx - 2;
-    const int i0 = 1 + threadIdx.x * E;
-    const double A = G.rdx2 * G.dt;
-
-    double ynew[E];  // this thread's new values of the previous row (pre-nudge)
-    for (int k = 1; k <= G.nz - 2; k++) {
-        const double Bk = mpar ? G.rdz2 * G.dt / rhow[k] : G.rdz2 * G.dt / rhou[k];
-        double a[E], e[E];
-        Affine m{1., 0.};
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            const int i = i0 + j;
-            a[j] = 0.;
-            e[j] = 0.;
-            if (i > n) continue;
-            const double old = f[ix(G, i, k)], oldE = f[ix(G, i + 1, k)], oldT = f[ix(G, i, k + 1)];
-            const double yB = (k == 1) ? f[ix(G, i, 0)] : ynew[j];
-            double Ke, Kw, Ct, Cb;
-            if (mpar) {
-                Ke = 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
-                Kw = 0.5 * (K[ix(G, i - 1, k)] + K[ix(G, i - 1, k - 1)]);
-                Ct = rhou[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
-                Cb = rhou[k - 1] * 0.5 * (K[ix(G, i, k - 1)] + K[ix(G, i - 1, k - 1)]);
-            } else {
-                Ke = 0.5 * (K[ix(G, i + 1, k)] + K[ix(G, i, k)]);
-                Kw = 0.5 * (K[ix(G, i, k)] + K[ix(G, i - 1, k)]);
-                Ct = rhow[k + 1] * 0.5 * (K[ix(G, i, k + 1)] + K[ix(G, i, k)]);
-                Cb = rhow[k] * 0.5 * (K[ix(G, i, k)] + K[ix(G, i, k - 1)]);
-            }
-            a[j] = A * Kw;
-            e[j] = old + A * (Ke * (oldE - old) - Kw * old) + Bk * (Ct * (oldT - old) - Cb * (old - yB));
-            m = after(Affine{a[j], e[j]}, m);
-        }
-        const double y0 = f[ix(G, 0, k)];  // ghost: old value, as in the reference
-        const Affine pre = block_exclusive_scan(m);
-        double y = pre.a * y0 + pre.b;
-        const double r = rate ? rate[k] : 0.;
-#pragma unroll
-        for (int j = 0; j < E; j++) {
-            const int i = i0 + j;
-            if (i > n) continue;
-            y = a[j] * y + e[j];
-            ynew[j] = y;
-            double out = y;
-            if (r != 0.) out = (fld == 1) ? y - r * (y - thb_init[k]) : y - r * y;
-            f[ix(G, i, k)] = out;
-        }
-        __syncthreads();
-    }
 }
 
 // NumericalProcess::Nudge_qv (NumericalProcess.cpp:86-95), all points.
@@ -300,18 +180,10 @@ __global__ void __launch_bounds__(256) k_update_mean(Grid G, StepPtrs S, int shi
 
 void launch_kessler(const Grid &G, const StepPtrs &S, const RedSlot &rs, hipStream_t st) {
     hipLaunchKernelGGL(k_kessler, grid2d(G), dim3(BX, BY), 0, st, G, S, rs);
+    finalize<6>(rs, nblocks(grid2d(G)), S.s, RED_NEGVAL, S_NEG_RATIO, st);
 }
 void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st) {
     hipLaunchKernelGGL(k_turb_coeffs, grid2d(G), dim3(BX, BY), 0, st, G, S);
-}
-bool diffusion_supported(const Grid &G) { return G.nx - 2 <= 8 * DT; }
-
-void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
-    const int n = G.nx - 2;
-    if (n <= DT) hipLaunchKernelGGL(k_diffusion<1>, dim3(5), dim3(DT), 0, st, G, S, nudge);
-    else if (n <= 2 * DT) hipLaunchKernelGGL(k_diffusion<2>, dim3(5), dim3(DT), 0, st, G, S, nudge);
-    else if (n <= 4 * DT) hipLaunchKernelGGL(k_diffusion<4>, dim3(5), dim3(DT), 0, st, G, S, nudge);
-    else hipLaunchKernelGGL(k_diffusion<8>, dim3(5), dim3(DT), 0, st, G, S, nudge);
 }
 
 // Nudge_theta and Nudge_zeta alone (NumericalProcess.cpp:60-84), pointwise;
@@ -335,4 +207,4 @@ void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipSt
     hipLaunchKernelGGL(k_update_mean, dim3(G.nz - 2), dim3(256), 0, st, G, S, int(shift_ubar));
 }
 
-}  // namespace vvm
+}  // namespace vvmd

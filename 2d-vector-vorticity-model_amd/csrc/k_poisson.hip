@@ -17,7 +17,7 @@
 
 #include "poisson.h"
 
-namespace vvm {
+namespace vvmd {
 
 namespace {
 
@@ -31,116 +31,6 @@ __device__ inline T applyL(const T *__restrict__ x, int N, int i, int j, T cxj, 
     const size_t r = size_t(j) * N;
     const T c = x[r + i];
     return cxj * (T(2) * c - x[r + im] - x[r + ip]) + cuj * (c - x[r + N + i]) + cujm * (c - x[r - N + i]);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_jacobi_first(int N, int M, const T *b, T *x, const T *dinv, T omega) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    if (i >= N || j > M) return;
-    const size_t c = size_t(j) * N + i;
-    x[c] = omega * dinv[j] * b[c];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_jacobi(int N, int M, const T *b, const T *xin, T *xout, const T *cx,
-                                                  const T *cu, const T *dinv, T omega) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    if (i >= N || j > M) return;
-    const size_t c = size_t(j) * N + i;
-    xout[c] = xin[c] + omega * dinv[j] * (b[c] - applyL(xin, N, i, j, cx[j], cu[j], cu[j - 1]));
-}
-
-// Last post-smoothing sweep on level 0 fused with the CG dot product r.z.
-template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_jacobi_dot(int N, int M, const T *b, const T *xin, T *xout, const T *cx,
-                                                      const T *cu, const T *dinv, T omega, double *sc, RedSlot rs,
-                                                      int init) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    double part[1] = {0.};
-    if (i < N && j <= M) {
-        const size_t c = size_t(j) * N + i;
-        const T z = xin[c] + omega * dinv[j] * (b[c] - applyL(xin, N, i, j, cx[j], cu[j], cu[j - 1]));
-        xout[c] = z;
-        part[0] = double(b[c]) * double(z);
-    }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            if (!init) sc[P_BETA] = tot[0] / sc[P_RZ];
-            sc[P_RZ] = tot[0];
-        }
-    }
-}
-
-// Coarse right-hand side: b_c = P^T (b_f - L_f x_f).
-template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_resid_restrict(int Nf, int Mf, const T *bf, const T *xf, const T *cx,
-                                                          const T *cu, int Nc, int Mc, T *bc) {
-    const int ic = blockIdx.x * LX + threadIdx.x, jc = 1 + blockIdx.y * LY + threadIdx.y;
-    if (ic >= Nc || jc > Mc) return;
-    T s = 0;
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++) {
-        const int jf = 2 * jc + dj;
-        const T wj = (dj == 0) ? T(1) : T(0.5);
-#pragma unroll
-        for (int di = -1; di <= 1; di++) {
-            int i_f = 2 * ic + di;
-            if (i_f < 0) i_f += Nf;
-            if (i_f >= Nf) i_f -= Nf;
-            const T wi = (di == 0) ? T(1) : T(0.5);
-            const T r = bf[size_t(jf) * Nf + i_f] - applyL(xf, Nf, i_f, jf, cx[jf], cu[jf], cu[jf - 1]);
-            s += wi * wj * r;
-        }
-    }
-    bc[size_t(jc) * Nc + ic] = s;
-}
-
-// x_f += P x_c (bilinear; coarse rows 0 and Mc+1 are zero).
-template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_prolong_add(int Nf, int Mf, T *xf, int Nc, const T *xc) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    if (i >= Nf || j > Mf) return;
-    const int ic = i >> 1, jc = j >> 1;
-    const int ic1 = (ic + 1 == Nc) ? 0 : ic + 1;
-    const T *r0 = xc + size_t(jc) * Nc, *r1 = r0 + Nc;
-    T v;
-    if (!(i & 1) && !(j & 1)) v = r0[ic];
-    else if ((i & 1) && !(j & 1)) v = T(0.5) * (r0[ic] + r0[ic1]);
-    else if (!(i & 1)) v = T(0.5) * (r0[ic] + r1[ic]);
-    else v = T(0.25) * (r0[ic] + r0[ic1] + r1[ic] + r1[ic1]);
-    xf[size_t(j) * Nf + i] += v;
-}
-
-// Coarsest level: `iters` damped-Jacobi sweeps from zero in one workgroup,
-// the iterate held in LDS (at most kCoarseMax points).
-constexpr int kCoarseMax = 2048;  // N * (M + 2) points, both buffers in LDS
-
-template <typename T>
-__global__ void __launch_bounds__(1024) k_coarse_solve(int N, int M, const T *b, T *x, const T *cx, const T *cu,
-                                                       const T *dinv, T omega, int iters) {
-    __shared__ T xs[2][kCoarseMax];
-    const int n = N * M;
-    const int tot = N * (M + 2);
-    for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        xs[0][t] = 0;
-        xs[1][t] = 0;
-    }
-    __syncthreads();
-    int cur = 0;
-    for (int it = 0; it < iters; it++) {
-        for (int t = threadIdx.x; t < n; t += blockDim.x) {
-            const int j = 1 + t / N, i = t % N;
-            const size_t c = size_t(j) * N + i;
-            xs[cur ^ 1][c] = xs[cur][c] + omega * dinv[j] * (b[c] - applyL(xs[cur], N, i, j, cx[j], cu[j], cu[j - 1]));
-        }
-        __syncthreads();
-        cur ^= 1;
-    }
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const size_t c = size_t(1 + t / N) * N + t % N;
-        x[c] = xs[cur][c];
-    }
 }
 
 // ------------------------------------------------------------------ CG
@@ -163,13 +53,7 @@ __global__ void __launch_bounds__(LX *LY) k_pcg_pq(int N, int M, const T *z, con
         q[r + i] = Lp;
         part[0] = double(c) * double(Lp);
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            sc[P_PQ] = tot[0];
-            sc[P_ALPHA] = sc[P_RZ] / tot[0];
-        }
-    }
+    block_partials<1>(part, rs);
 }
 
 // x += alpha p, r -= alpha q, first smoothing sweep z = omega D^-1 r, and
@@ -185,13 +69,10 @@ __global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r,
         x[c] = x[c] + alpha * p[c];
         const T rn = r[c] - alpha * q[c];
         r[c] = rn;
-        z[c] = omega * dinv[j] * rn;
         const double wr = double(wD[j]) * double(rn);
         part[0] = wr * wr;
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot))
-        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_RNORM2] = tot[0];
+    block_partials<1>(part, rs);
 }
 
 // r = f - L x and ||w r||^2 into sc[slot] (and sc[slot2] if >= 0).
@@ -208,13 +89,7 @@ __global__ void __launch_bounds__(LX *LY) k_residual(int N, int M, const TF *f, 
         const double wr = double(wD[j]) * double(rv);
         part[0] = wr * wr;
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            sc[slot] = tot[0];
-            if (slot2 >= 0) sc[slot2] = tot[0];
-        }
-    }
+    block_partials<1>(part, rs);
 }
 
 // FP64 residual -> FP32 right-hand side of the correction equation, with
@@ -231,13 +106,7 @@ __global__ void __launch_bounds__(LX *LY) k_to_fp32(int N, int M, const double *
         const double wr = double(wD[j]) * double(v);
         part[0] = wr * wr;
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot)) {
-        if (threadIdx.x == 0 && threadIdx.y == 0) {
-            sc32[P_RNORM2] = tot[0];
-            sc32[P_RNORM2_0] = tot[0];
-        }
-    }
+    block_partials<1>(part, rs);
 }
 
 __global__ void __launch_bounds__(LX *LY) k_add_correction(int N, int M, double *x64, const float *e32) {
@@ -262,9 +131,7 @@ __global__ void __launch_bounds__(LX *LY) k_calw_rhs(Grid G, const double *zetap
         f[size_t(j) * N + ii] = -b / rhow[k];
         part[0] = b * b;
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot))
-        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_BNORM2] = tot[0];
+    block_partials<1>(part, rs);
 }
 
 __global__ void __launch_bounds__(LX *LY) k_rhs_from_b(int N, int M, const double *bref, const double *wD,
@@ -276,9 +143,7 @@ __global__ void __launch_bounds__(LX *LY) k_rhs_from_b(int N, int M, const doubl
         f[size_t(j) * N + ii] = -b / wD[j];
         part[0] = b * b;
     }
-    double tot[1];
-    if (grid_sum_last<1>(part, rs, tot))
-        if (threadIdx.x == 0 && threadIdx.y == 0) sc[P_BNORM2] = tot[0];
+    block_partials<1>(part, rs);
 }
 
 __global__ void k_x_ref(int N, int M, double *x, double *xref, int to_ref) {
@@ -505,7 +370,10 @@ bool build_hier(MGHier<T> &H, int N, int M, const std::vector<double> &cx0, cons
         M = Mc;
     }
     H.nlev = l;
-    return true;
+    // the one-workgroup kernel takes every level that fits in 64 KiB of LDS;
+    // the finest levels run as tiled down/up kernels
+    mg_plan(H, size_t(64) * 1024);
+    return H.coarse_from < H.nlev;
 }
 
 template <typename T>
@@ -524,69 +392,6 @@ void free_hier(MGHier<T> &H) {
     H.nlev = 0;
 }
 
-// V-cycle on level l >= 1 (x_l from b_l, starting from zero).
-template <typename T>
-void vcycle_coarse(MGHier<T> &H, int l, hipStream_t st) {
-    MGLevel<T> &L = H.lev[l];
-    const dim3 g = lgrid(L.N, L.M), b(LX, LY);
-    if (l == H.nlev - 1) {
-        if (size_t(L.N) * (L.M + 2) <= size_t(kCoarseMax)) {
-            hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(1024), 0, st, L.N, L.M, L.b, L.x, L.cx, L.cu, L.dinv,
-                               H.omega, H.coarse_iters);
-        } else {
-            hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.dinv, H.omega);
-            for (int s = 1; s < 2 * (H.nu1 + H.nu2); s++) {
-                hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
-                std::swap(L.x, L.t);
-            }
-        }
-        return;
-    }
-    hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.dinv, H.omega);
-    for (int s = 1; s < H.nu1; s++) {
-        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
-        std::swap(L.x, L.t);
-    }
-    MGLevel<T> &C = H.lev[l + 1];
-    hipLaunchKernelGGL(k_resid_restrict<T>, lgrid(C.N, C.M), b, 0, st, L.N, L.M, L.b, L.x, L.cx, L.cu, C.N, C.M, C.b);
-    vcycle_coarse(H, l + 1, st);
-    hipLaunchKernelGGL(k_prolong_add<T>, g, b, 0, st, L.N, L.M, L.x, C.N, C.x);
-    for (int s = 0; s < H.nu2; s++) {
-        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, L.N, L.M, L.b, L.x, L.t, L.cx, L.cu, L.dinv, H.omega);
-        std::swap(L.x, L.t);
-    }
-}
-
-// Level-0 V-cycle applied to P.r into P.zA.  The first smoothing sweep has
-// been done already when first_done (fused into k_pcg_update); the last
-// sweep computes r.z and the CG beta.
-template <typename T>
-void vcycle0(PCG<T> &P, bool first_done, bool init, const RedSlot &rs, hipStream_t st) {
-    MGHier<T> &H = P.mg;
-    MGLevel<T> &L = H.lev[0];
-    const dim3 g = lgrid(P.N, P.M), b(LX, LY);
-    if (!first_done) hipLaunchKernelGGL(k_jacobi_first<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, L.dinv, H.omega);
-    for (int s = 1; s < H.nu1; s++) {
-        hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega);
-        std::swap(P.zA, P.zB);
-    }
-    if (H.nlev > 1) {
-        MGLevel<T> &C = H.lev[1];
-        hipLaunchKernelGGL(k_resid_restrict<T>, lgrid(C.N, C.M), b, 0, st, P.N, P.M, P.r, P.zA, L.cx, L.cu, C.N, C.M,
-                           C.b);
-        vcycle_coarse(H, 1, st);
-        hipLaunchKernelGGL(k_prolong_add<T>, g, b, 0, st, P.N, P.M, P.zA, C.N, C.x);
-    }
-    for (int s = 0; s < H.nu2; s++) {
-        if (s == H.nu2 - 1)
-            hipLaunchKernelGGL(k_jacobi_dot<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega,
-                               P.sc, rs, int(init));
-        else
-            hipLaunchKernelGGL(k_jacobi<T>, g, b, 0, st, P.N, P.M, P.r, P.zA, P.zB, L.cx, L.cu, L.dinv, H.omega);
-        std::swap(P.zA, P.zB);
-    }
-}
-
 // Preconditioned CG iterations on P (x, r and sc[P_RNORM2] set up by the
 // caller) until ||w r||^2 <= target or max_iter.  The convergence test lags
 // one V-cycle behind the GPU so the stream never drains.  Returns the
@@ -595,7 +400,7 @@ template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
              hipEvent_t ev, hipStream_t st, double *rn2, bool *conv) {
     const dim3 g = lgrid(P.N, P.M), b(LX, LY);
-    vcycle0(P, false, true, rs, st);
+    mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st);
     int it = 0;
     double best = 1e300;
     int stall = 0;
@@ -604,12 +409,14 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
         it++;
         hipLaunchKernelGGL(k_pcg_pq<T>, g, b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
                            P.mg.lev[0].cu, P.sc, rs, int(it == 1));
+        finalize<1>(rs, nblocks(g), P.sc, RED_ALPHA, 0, st);
         std::swap(P.pA, P.pB);
         hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.zA, P.mg.lev[0].dinv,
                            P.wD, P.mg.omega, P.sc, rs);
+        finalize<1>(rs, nblocks(g), P.sc, RED_STORE, P_RNORM2, st);
         hipMemcpyAsync(hs, P.sc + P_RNORM2, sizeof(double), hipMemcpyDeviceToHost, st);
         hipEventRecord(ev, st);
-        if (it < max_iter) vcycle0(P, true, false, rs, st);
+        if (it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
         hipEventSynchronize(ev);
         const double rn = *hs;
         *rn2 = rn;
@@ -713,11 +520,13 @@ void Poisson::destroy() {
 
 void Poisson::rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st) {
     hipLaunchKernelGGL(k_calw_rhs, lgrid(N, M), dim3(LX, LY), 0, st, G, zetap, rhow_dev, f, pcg64.sc, red);
+    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st);
 }
 
 void Poisson::rhs_from_b(const double *b, hipStream_t st) {
     // the FP64 row weights wD[j] are rhow[j+1]
     hipLaunchKernelGGL(k_rhs_from_b, lgrid(N, M), dim3(LX, LY), 0, st, N, M, b, pcg64.wD, f, pcg64.sc, red);
+    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st);
 }
 
 void Poisson::x_to_ref(double *xref, hipStream_t st) {
@@ -757,6 +566,7 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     const dim3 g = lgrid(N, M), b(LX, LY);
     hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, P.x, P.r, P.mg.lev[0].cx, P.mg.lev[0].cu,
                        P.wD, P.sc, red, int(P_RNORM2), int(P_RNORM2_0));
+    finalize<1>(red, nblocks(g), P.sc, RED_STORE, P_RNORM2, st, P_RNORM2_0);
     hipMemcpyAsync(hscal, P.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
     hipStreamSynchronize(st);
     const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
@@ -791,6 +601,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
     for (int outer = 0; outer <= max_outer; outer++) {
         hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, D.x, r64, D.mg.lev[0].cx,
                            D.mg.lev[0].cu, D.wD, D.sc, red, int(P_RNORM2), -1);
+        finalize<1>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st);
         hipMemcpyAsync(hscal, D.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
         bn2 = hscal[P_BNORM2];
@@ -810,6 +621,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         if (outer == max_outer) break;
         R.refine++;
         hipLaunchKernelGGL(k_to_fp32, g, b, 0, st, N, M, r64, F.r, F.x, F.wD, F.sc, red);
+        finalize<1>(red, nblocks(g), F.sc, RED_STORE, P_RNORM2, st, P_RNORM2_0);
         hipMemcpyAsync(hscal, F.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
         hipStreamSynchronize(st);
         const double r0 = hscal[P_RNORM2];
@@ -831,4 +643,4 @@ void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st) {
     hipLaunchKernelGGL(k_calu_cols, dim3((N + 63) / 64), dim3(64, UC), 0, st, G, S);
 }
 
-}  // namespace vvm
+}  // namespace vvmd

@@ -1,0 +1,51 @@
+// mg.h -- multigrid hierarchy of the elliptic solve (see k_mg.hip).
+#pragma once
+
+#include "vvm_internal.h"
+
+namespace vvmd {
+
+template <typename T>
+struct MGLevel {
+    int N = 0, M = 0;
+    T *cx = nullptr, *cu = nullptr, *dinv = nullptr;  // [M+2], [M+1], [M+2]
+    T *x = nullptr, *b = nullptr, *t = nullptr;       // (M+2)*N each (levels >= 1)
+};
+
+constexpr int kMaxLevels = 16;
+constexpr int kMaxCoarse = 10;  // levels handled by the one-workgroup kernel
+
+// Levels coarse_from..nlev-1 run in one workgroup with LDS offsets (in T).
+template <typename T>
+struct CoarseParams {
+    int nlev = 0;
+    int N[kMaxCoarse] = {}, M[kMaxCoarse] = {};
+    const T *cx[kMaxCoarse] = {}, *cu[kMaxCoarse] = {}, *dinv[kMaxCoarse] = {};
+    int offB[kMaxCoarse] = {}, offX[kMaxCoarse] = {};
+    int offT = 0, Tsize = 0;
+    T omega = T(0.8);
+    int coarse_iters = 32;
+};
+
+template <typename T>
+struct MGHier {
+    int nlev = 0;
+    MGLevel<T> lev[kMaxLevels];
+    T omega = T(0.8);      // damped-Jacobi weight; two sweeps down, two up
+    int coarse_iters = 32;  // Jacobi iterations on the coarsest level
+    int coarse_from = 0;    // first level of the one-workgroup kernel
+    size_t coarse_lds = 0;  // its LDS bytes
+    CoarseParams<T> cp;
+};
+
+// Choose coarse_from for an LDS budget (bytes); fills cp.
+template <typename T>
+int mg_plan(MGHier<T> &H, size_t lds_budget);
+
+// One V-cycle on r (level-0 right-hand side) into zA (zA/zB are swapped as
+// the result moves between buffers); also forms r.z and the CG beta in sc
+// (init: only r.z).
+template <typename T>
+void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st);
+
+}  // namespace vvmd

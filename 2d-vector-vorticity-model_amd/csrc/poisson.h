@@ -12,30 +12,10 @@
 // held at zero (the Dirichlet rows); N = nx-2, M = nz-3.
 #pragma once
 
-#include "vvm_internal.h"
+#include "mg.h"
 
-namespace vvm {
+namespace vvmd {
 
-template <typename T>
-struct MGLevel {
-    int N = 0, M = 0;
-    T *cx = nullptr, *cu = nullptr, *dinv = nullptr;  // [M+2], [M+1], [M+2]
-    T *x = nullptr, *b = nullptr, *t = nullptr;       // (M+2)*N each
-};
-
-constexpr int kMaxLevels = 16;
-
-template <typename T>
-struct MGHier {
-    int nlev = 0;
-    MGLevel<T> lev[kMaxLevels];
-    T omega = T(0.8);
-    int nu1 = 2, nu2 = 2;
-    int coarse_iters = 64;
-};
-
-// PCG device scalars (doubles).
-enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, NPSCAL = 8 };
 
 // One CG instance in precision T (vectors on level 0 of its hierarchy).
 template <typename T>
@@ -107,4 +87,4 @@ class Poisson {
 bool cal_u_supported(const Grid &G);  // nx - 2 <= 8192
 void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st);
 
-}  // namespace vvm
+}  // namespace vvmd

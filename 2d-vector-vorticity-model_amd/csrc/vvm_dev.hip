@@ -13,7 +13,7 @@
 #include "poisson.h"
 #include "vvm_internal.h"
 
-using namespace vvm;
+using namespace vvmd;
 
 struct vvm_dev {
     int device = 0;
@@ -30,6 +30,7 @@ struct vvm_dev {
     double *evap = nullptr, *accr = nullptr, *autoc = nullptr, *cond = nullptr;
     double *precip = nullptr, *xi = nullptr, *uxi = nullptr;
     double *s = nullptr;
+    double *dcoef[DC_N] = {};
     Profiles P{};
     Reductions red{};
 
@@ -136,6 +137,7 @@ StepPtrs ptrs(vvm_dev *h) {
     S.uxi = h->uxi;
     S.s = h->s;
     S.P = h->P;
+    for (int q = 0; q < DC_N; q++) S.dcoef[q] = h->dcoef[q];
     return S;
 }
 
@@ -310,6 +312,7 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     h->xi = A(G.nx);
     h->uxi = A(G.nx);
     h->s = A(NSCALAR);
+    for (int q = 0; q < DC_N; q++) h->dcoef[q] = A(n2);
     double **prof[] = {&h->P.rhou, &h->P.rhow, &h->P.pib, &h->P.pb, &h->P.thb, &h->P.thbm, &h->P.thb_init,
                        &h->P.thb_zeta, &h->P.thvb, &h->P.thvbm, &h->P.qvb, &h->P.qvb0, &h->P.qvsb, &h->P.z,
                        &h->P.z_zeta, &h->P.lambda2, &h->P.Q1LS, &h->P.Q2LS, &h->P.vtfac, &h->P.pcoef,
@@ -321,7 +324,6 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     for (RedSlot *rs : slots) {
         rs->max_blocks = maxb;
         rs->partial = A(size_t(8) * maxb);
-        rs->counter = reinterpret_cast<unsigned *>(A(1));
     }
     for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     if (!ok) {
@@ -611,6 +613,9 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
     HIPCHK(h, hipStreamSynchronize(h->st));
     const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     kt_collect(h);
+    double derr = 0.;
+    HIPCHK(h, hipMemcpy(&derr, h->s + S_DIFF_ERR, sizeof(double), hipMemcpyDeviceToHost));
+    if (derr != 0.) return fail(h, VVM_ERR_STATE, "diffusion sweep: a wavefront gave up waiting for its carry");
     if (stats) {
         *stats = acc;
         stats->steps = nsteps;

@@ -5,7 +5,7 @@
 #include "dev_common.h"
 #include "vvm_dev.h"
 
-namespace vvm {
+namespace vvmd {
 
 // Field slots of the prognostic triplets (rotated instead of copied).
 enum { F_ZETA = 0, F_TH, F_QV, F_QC, F_QR, NPROG };
@@ -23,10 +23,22 @@ struct Profiles {
 };
 
 // Device scalars (one double each, in one small block).
-enum { S_UBARTOP = 0, S_UBARTOPP, S_UBARTOPM, S_DUBAR0, S_DUBAR1, S_NEG_RATIO, S_PRECIP_FAC = 9, NSCALAR = 16 };
+enum {
+    S_UBARTOP = 0,
+    S_UBARTOPP,
+    S_UBARTOPM,
+    S_DUBAR0,
+    S_DUBAR1,
+    S_NEG_RATIO,       // 3 slots
+    S_PRECIP_FAC = 9,  // pow(rhou[1]/rhow[1], -0.5)
+    S_DIFF_ERR = 10,   // set to 1 if a diffusion wave gave up waiting
+    NSCALAR = 16
+};
 
 // Ratios of NegativeValueProcess (3 fields) live at s[S_NEG_RATIO + f]:
 // a value < 0 means "no rescale" (the reference's early return).
+
+enum { DC_AM = 0, DC_DM, DC_AH, DC_DH, DC_C0, DC_N = DC_C0 + NPROG };  // DC_C0 + field
 
 // Pointers used by one timestep (the rotation resolved).
 struct StepPtrs {
@@ -37,6 +49,10 @@ struct StepPtrs {
     double *precip, *xi, *uxi;
     double *s;  // device scalars
     Profiles P;
+    // diffusion-sweep coefficients (new = a yW + c + d yB), formed in
+    // parallel before the sequential sweep: a, d of Mparam (zeta) and of
+    // Hparam (shared by th, qv, qc, qr), and c of each field.
+    double *dcoef[DC_N];
 };
 
 struct Reductions {
@@ -66,4 +82,4 @@ void launch_nudge_qv(const Grid &G, const StepPtrs &S, double tau, hipStream_t s
 void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipStream_t st);
 
 // Elliptic (k_poisson.hip): see poisson.h
-}  // namespace vvm
+}  // namespace vvmd

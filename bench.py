@@ -95,8 +95,27 @@ def cpu_baseline(case: V.Case, state: V.HostState) -> dict | None:
             "sample": f"{n} steps of the 2048x256 workload, C restatement (direct FFT+Thomas solve), 1 thread"}
 
 
+def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> None:
+    """A device-to-device copy of known size (the runtime's blit kernel), so
+    that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    for _ in range(reps):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"[bench] copy calibration: {reps} x {nbytes} bytes device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
@@ -114,6 +133,8 @@ def main():
         "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
     }
     wanted = [p for p in args.paths.split(",") if p]
+    if args.calibrate_copy:
+        copy_calibration()
     base = V.case("c2")
     state = V.init_state(base)
     pts = (base.cfg.nx - 2) * (base.cfg.nz - 2)

@@ -129,3 +129,33 @@ def test_poisson_isolated(oracle, mode):
     assert rr <= 1e-10, rr
     assert s.rel_residual <= 1e-10
     assert rel_err(x, xe) <= 1e-6
+
+
+@pytest.mark.parametrize("nx_int", [128, 1000, 2048, 3000])
+def test_diffusion_sweep_wide_grids(oracle, nx_int):
+    """The in-place sweep at widths that use 1..16 wavefronts and 1, 2 or 4
+    columns per lane (the cross-wave carries of the device pipeline)."""
+    c = V.grid_case(nx_int, 30, dx=100.0, dt=1.0)
+    st = perturbed_state(c, seed=nx_int, step=5)
+    for f in ("zeta", "th", "qv", "qc", "qr"):
+        st[f + "p"][:] = st[f]
+    st["qcp"][:] = np.abs(st["qcp"])
+    want = st.copy()
+    oracle.op("turbulence", c.cfg, want)
+    with V.Device(c.cfg) as dev:
+        got = gpu_run_op(dev, st, "turbulence")
+    assert_fields_close(got, want, ["RKM", "RKH", "zetap", "thp", "qvp", "qcp", "qrp"], 1e-12,
+                        where=f"turbulence {nx_int}")
+
+
+def test_one_step_parity_wide(oracle):
+    c = V.grid_case(2048, 64, dx=100.0, dt=1.0)
+    st = V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 2)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.step(2)
+        got = st.copy()
+        dev.download(got)
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="2048x64 2 steps")
