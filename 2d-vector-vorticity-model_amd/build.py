@@ -53,9 +53,34 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_host()
     if verbose:
-        print(f"built {OUT} from {len(srcs)} sources")
+        print(f"built {OUT} from {len(srcs)} sources, {HOST_LIB} and {HOST_MAIN}")
     return OUT
+
+
+HOST_LIB = os.path.join(PKG, "libvvm.so")
+HOST_MAIN = os.path.join(PKG, "vvm_main")
+
+
+def build_host() -> None:
+    """The C++ state/config API (host/vvm.cpp -> libvvm.so) and the model
+    driver (host/vvm_main.cpp -> vvm_main), both plain host code over
+    libvvm_amd.so."""
+    cxx = os.environ.get("CXX", "g++")
+    base = [cxx, "-O2", "-std=c++17", "-Wall", f"-I{ROOT}/include"]
+    link = [f"-L{PKG}", "-lvvm_amd", "-Wl,-rpath,$ORIGIN"]
+    steps = ((base + ["-fPIC", "-shared", os.path.join(PKG, "host", "vvm.cpp"), "-o", HOST_LIB + ".tmp"] + link,
+              HOST_LIB),
+             (base + [os.path.join(PKG, "host", "vvm_main.cpp"), "-o", HOST_MAIN + ".tmp", "-lvvm"] + link,
+              HOST_MAIN))
+    for cmd, target in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        if r.stderr.strip():
+            sys.stderr.write(r.stderr)
+        os.replace(target + ".tmp", target)
 
 
 if __name__ == "__main__":

@@ -159,9 +159,11 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
 // All levels from Lc down, in one workgroup with every array in LDS.
 template <typename T>
 __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T *__restrict__ bin,
-                                                    T *__restrict__ xout) {
+                                                    T *__restrict__ xout, T *ws) {
     extern __shared__ unsigned char lds_raw[];
-    T *lds = reinterpret_cast<T *>(lds_raw);
+    // ws: global workspace of the same layout when the levels exceed LDS
+    // (__syncthreads orders global memory within the workgroup as well)
+    T *lds = ws ? ws : reinterpret_cast<T *>(lds_raw);
     const T om = cp.omega;
     const int L = cp.nlev;
     auto B = [&](int l) { return lds + cp.offB[l]; };
@@ -296,9 +298,12 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
         if ((tot + tmax) * sizeof(T) > lds_budget || H.nlev - lc > kMaxCoarse) break;
         H.coarse_from = lc;
     }
+    // grids whose z size stops the coarsening early: the coarsest level
+    // alone, from global memory
+    const bool global = (H.coarse_from >= H.nlev);
+    if (global) H.coarse_from = H.nlev - 1;
     CoarseParams<T> &cp = H.cp;
     cp = CoarseParams<T>();
-    if (H.coarse_from >= H.nlev) return 0;
     cp.nlev = H.nlev - H.coarse_from;
     int off = 0, tmax = 0;
     for (int q = 0; q < cp.nlev; q++) {
@@ -320,6 +325,13 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
     cp.omega = H.omega;
     cp.coarse_iters = H.coarse_iters;
     H.coarse_lds = size_t(off + tmax) * sizeof(T);
+    if (global) {
+        if (hipMalloc(reinterpret_cast<void **>(&H.cws), H.coarse_lds) != hipSuccess) {
+            H.cws = nullptr;
+            return -1;
+        }
+        H.coarse_lds = 0;
+    }
     return 0;
 }
 
@@ -339,7 +351,7 @@ void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doub
     }
     if (Lc < L) {
         T *xo = (Lc == 0) ? zA : H.lev[Lc].x;
-        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), H.coarse_lds, st, H.cp, bptr(Lc), xo);
+        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), H.coarse_lds, st, H.cp, bptr(Lc), xo, H.cws);
     }
     const bool dot_on_up = (Lc > 0);
     for (int l = std::min(Lc, L) - 1; l >= 0; l--) {

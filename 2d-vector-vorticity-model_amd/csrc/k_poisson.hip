@@ -372,8 +372,7 @@ bool build_hier(MGHier<T> &H, int N, int M, const std::vector<double> &cx0, cons
     H.nlev = l;
     // the one-workgroup kernel takes every level that fits in 64 KiB of LDS;
     // the finest levels run as tiled down/up kernels
-    mg_plan(H, size_t(64) * 1024);
-    return H.coarse_from < H.nlev;
+    return mg_plan(H, size_t(64) * 1024) == 0;
 }
 
 template <typename T>
@@ -389,6 +388,8 @@ void free_hier(MGHier<T> &H) {
             hipFree(L.t);
         }
     }
+    if (H.cws) hipFree(H.cws);
+    H.cws = nullptr;
     H.nlev = 0;
 }
 

@@ -34,11 +34,15 @@ struct MGHier {
     T omega = T(0.8);      // damped-Jacobi weight; two sweeps down, two up
     int coarse_iters = 32;  // Jacobi iterations on the coarsest level
     int coarse_from = 0;    // first level of the one-workgroup kernel
-    size_t coarse_lds = 0;  // its LDS bytes
+    size_t coarse_lds = 0;  // its LDS bytes (0: the levels live in cws)
+    T *cws = nullptr;       // global-memory workspace of the one-workgroup
+                            // kernel when even the coarsest level exceeds LDS
     CoarseParams<T> cp;
 };
 
-// Choose coarse_from for an LDS budget (bytes); fills cp.
+// Choose coarse_from for an LDS budget (bytes); fills cp.  When no level
+// fits, the coarsest runs from cws (allocated here).  Returns 0, or -1 if
+// that allocation fails.
 template <typename T>
 int mg_plan(MGHier<T> &H, size_t lds_budget);
 

@@ -1,0 +1,440 @@
+// vvm.cpp -- the C++ state/config API (include/vvm.hpp) over the C ABI of
+// include/vvm_dev.h.  Host arrays keep the reference's layout and names
+// (src/Declare.hpp); the timestep itself runs on the device.
+#include "vvm.hpp"
+
+#include <sys/stat.h>
+
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+#include <string>
+#include <utility>
+#include <vector>
+
+Config_VVM::Config_VVM(double dt, double dx, double dz, int XRANGE, int ZRANGE, double TIMEEND, int TIMEROUTPUTSIZE,
+                       std::string outputpath, int OUTPUTSTEP, double Kx, double Kz, double TIMETS, double tolerance,
+                       double GRAVITY, double Cp, double Cv, double Rd, double Lv, double P0, double PSURF,
+                       double addforcingtime, int CASE, double mositure_nudge_time, int year, int month, int day,
+                       double hour, double minute, double second, double lon, double lat)
+    : dt(dt), dx(dx), dz(dz),
+      // the reference adds one ghost cell on each side (src/Declare.hpp:33)
+      XRANGE(int(XRANGE + 2 * dx)), ZRANGE(int(ZRANGE + 2 * dz)),
+      TIMEEND(TIMEEND), TIMEROUTPUTSIZE(TIMEROUTPUTSIZE), outputpath(std::move(outputpath)), OUTPUTSTEP(OUTPUTSTEP),
+      Kx(Kx), Kz(Kz), TIMETS(TIMETS), tolerance(tolerance), GRAVITY(GRAVITY), Cp(Cp), Cv(Cv), Rd(Rd), Lv(Lv), P0(P0),
+      PSURF(PSURF), addforcingtime(addforcingtime), CASE(CASE), mositure_nudge_time(mositure_nudge_time), year(year),
+      month(month), day(day), hour(hour), minute(minute), second(second), lon(lon), lat(lat) {}
+
+namespace {
+
+[[noreturn]] void die(const vvm &m, int rc, const char *what) {
+    std::fprintf(stderr, "vvm: %s failed (%d): %s\n", what, rc, m.dev ? vvm_dev_last_error(m.dev) : "no device");
+    std::exit(1);
+}
+
+void check(const vvm &m, int rc, const char *what) {
+    if (rc < 0) die(m, rc, what);
+}
+
+void ensure_device(vvm &m) {
+    if (!m.dev) vvm::PoissonSolver::InitPoissonMatrix(m);
+    m.sync_to_device();
+}
+
+// One operator on the device with the host arrays current before and after.
+void run_op(vvm &m, int op, const char *name) {
+    ensure_device(m);
+    int rc = vvm_dev_op(m.dev, op);
+    if (rc > 0) std::printf("%s: the elliptic solve did not reach its stop rule\n", name);
+    check(m, rc, name);
+    m.host_stale = true;
+    m.sync_to_host();
+}
+
+bool is_output_step(const vvm &m, int step) {
+    // src/Iteration.cpp:231
+    return step % m.OUTPUTSTEP == 0 || step == m.TIMEEND - 1 || step == m.TIMEEND - 2 || step == 550001;
+}
+
+}  // namespace
+
+vvm::vvm(const Config_VVM &c)
+    : rdx(1.0 / c.dx), r2dx(rdx / 2.0), rdz(1.0 / c.dz), r2dz(rdz / 2.0), rdx2(rdx * rdx), rdz2(rdz * rdz),
+      nx(int(c.XRANGE / c.dx)), nz(int(c.ZRANGE / c.dz)), dt(c.dt), d2t(2.0 * c.dt), dx(c.dx), dz(c.dz),
+      XRANGE(c.XRANGE), ZRANGE(c.ZRANGE), TIMEEND(c.TIMEEND), TIMEROUTPUTSIZE(c.TIMEROUTPUTSIZE),
+      outputpath(c.outputpath), OUTPUTSTEP(c.OUTPUTSTEP), Kx(c.Kx), Kz(c.Kz), TIMETS(c.TIMETS),
+      tolerance(c.tolerance), GRAVITY(c.GRAVITY), Cp(c.Cp), Cv(c.Cv), Rd(c.Rd), Lv(c.Lv), P0(c.P0), PSURF(c.PSURF),
+      addforcingtime(c.addforcingtime), CASE(c.CASE), year(c.year), month(c.month), day(c.day), hour(c.hour),
+      minute(c.minute), second(c.second), lon(c.lon), lat(c.lat), moisture_nudge_time(c.mositure_nudge_time) {
+    dev_config.nx = nx;
+    dev_config.nz = nz;
+    dev_config.dx = dx;
+    dev_config.dz = dz;
+    dev_config.dt = dt;
+    dev_config.GRAVITY = GRAVITY;
+    dev_config.Cp = Cp;
+    dev_config.Cv = Cv;
+    dev_config.Rd = Rd;
+    dev_config.Lv = Lv;
+    dev_config.P0 = P0;
+    dev_config.PSURF = PSURF;
+    dev_config.addforcingtime = addforcingtime;
+    dev_config.moisture_nudge_time = moisture_nudge_time;
+    dev_config.tolerance = tolerance;
+    dev_config.CASE = CASE;
+    dev_config.tropical_forcing = c.tropical_forcing;
+    dev_config.solver_mode = c.solver_mode;
+    dev_config.guess_mode = c.guess_mode;
+    dev_config.stop_mode = c.stop_mode;
+    dev_config.max_iter = c.max_iter;
+    dev_config.tol_rel = c.tol_rel;
+    gpu_id = c.gpu_id;
+    allocateMemory();
+}
+
+vvm::~vvm() {
+    if (dev) vvm_dev_destroy(dev);
+    dev = nullptr;
+    deallocateMemory();
+}
+
+double **vvm::allocate2DContinuousArray(int rows, int cols, double *&contMemory) {
+    double **array = new double *[rows]();
+    contMemory = new double[size_t(rows) * cols]();
+    for (int i = 0; i < rows; ++i) array[i] = &contMemory[size_t(i) * cols];
+    return array;
+}
+
+void vvm::deallocate2DContinuousArray(double **array, double *contMemory) {
+    if (array != nullptr) {
+        delete[] contMemory;
+        delete[] array;
+    }
+}
+
+double ***vvm::allocate3DContinuousArray(int dim1, int dim2, int dim3, double *&contMemory) {
+    double ***array = new double **[dim1]();
+    contMemory = new double[size_t(dim1) * dim2 * dim3]();
+    for (int i = 0; i < dim1; ++i) {
+        array[i] = new double *[dim2]();
+        for (int j = 0; j < dim2; ++j) array[i][j] = &contMemory[(size_t(i) * dim2 + j) * dim3];
+    }
+    return array;
+}
+
+void vvm::deallocate3DContinuousArray(double ***array, double *contMemory) {
+    if (array != nullptr) {
+        delete[] contMemory;
+        for (int i = 0; i < nx; ++i) delete[] array[i];
+        delete[] array;
+    }
+}
+
+void vvm::allocateMemory() {
+    double **profiles[] = {&thb, &thb_init, &thbm, &thb_zeta, &rhou, &rhow, &pib, &qvb, &qvb0,
+                           &qvsb, &pb, &thvb, &thvbm, &z, &z_zeta, &lambda2, &Q1LS, &Q2LS};
+    for (double **p : profiles) *p = new double[nz]();
+    xi = new double[nx]();
+    uxi = new double[nx]();
+    precip = new double[nx]();
+    std::pair<double ***, double **> f2[] = {
+        {&zetap, &zetapcont}, {&zeta, &zetacont}, {&zetam, &zetamcont}, {&thp, &thpcont}, {&th, &thcont},
+        {&thm, &thmcont}, {&qvp, &qvpcont}, {&qv, &qvcont}, {&qvm, &qvmcont}, {&qcp, &qcpcont},
+        {&qc, &qccont}, {&qcm, &qcmcont}, {&qrp, &qrpcont}, {&qr, &qrcont}, {&qrm, &qrmcont},
+        {&u, &ucont}, {&w, &wcont}, {&RKM, &RKMcont}, {&RKH, &RKHcont}, {&evaporation, &evaporationcont},
+        {&accretion, &accretioncont}, {&autoconversion, &autoconversioncont},
+        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont}};
+    for (auto &f : f2) *f.first = allocate2DContinuousArray(nx, nz, *f.second);
+    std::pair<double ****, double **> f3[] = {
+        {&dth_advect, &dth_advectcont}, {&dth_buoyancy, &dth_buoyancycont}, {&dzeta_advect, &dzeta_advectcont},
+        {&dqv_advect, &dqv_advectcont}, {&dqc_advect, &dqc_advectcont}, {&dqr_advect, &dqr_advectcont},
+        {&dqr_VT, &dqr_VTcont}};
+    for (auto &f : f3) *f.first = allocate3DContinuousArray(nx, nz, 2, *f.second);
+    double **timers[] = {&t_advection, &t_poisson, &t_diffusion, &t_microphysics, &t_all};
+    for (double **t : timers) *t = new double[TIMEROUTPUTSIZE > 0 ? TIMEROUTPUTSIZE : 1]();
+}
+
+void vvm::deallocateMemory() {
+    double **profiles[] = {&thb, &thb_init, &thbm, &thb_zeta, &rhou, &rhow, &pib, &qvb, &qvb0, &qvsb, &pb,
+                           &thvb, &thvbm, &z, &z_zeta, &lambda2, &Q1LS, &Q2LS, &xi, &uxi, &precip,
+                           &t_advection, &t_poisson, &t_diffusion, &t_microphysics, &t_all};
+    for (double **p : profiles) {
+        delete[] *p;
+        *p = nullptr;
+    }
+    std::pair<double ***, double **> f2[] = {
+        {&zetap, &zetapcont}, {&zeta, &zetacont}, {&zetam, &zetamcont}, {&thp, &thpcont}, {&th, &thcont},
+        {&thm, &thmcont}, {&qvp, &qvpcont}, {&qv, &qvcont}, {&qvm, &qvmcont}, {&qcp, &qcpcont},
+        {&qc, &qccont}, {&qcm, &qcmcont}, {&qrp, &qrpcont}, {&qr, &qrcont}, {&qrm, &qrmcont},
+        {&u, &ucont}, {&w, &wcont}, {&RKM, &RKMcont}, {&RKH, &RKHcont}, {&evaporation, &evaporationcont},
+        {&accretion, &accretioncont}, {&autoconversion, &autoconversioncont},
+        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont}};
+    for (auto &f : f2) {
+        deallocate2DContinuousArray(*f.first, *f.second);
+        *f.first = nullptr;
+        *f.second = nullptr;
+    }
+    std::pair<double ****, double **> f3[] = {
+        {&dth_advect, &dth_advectcont}, {&dth_buoyancy, &dth_buoyancycont}, {&dzeta_advect, &dzeta_advectcont},
+        {&dqv_advect, &dqv_advectcont}, {&dqc_advect, &dqc_advectcont}, {&dqr_advect, &dqr_advectcont},
+        {&dqr_VT, &dqr_VTcont}};
+    for (auto &f : f3) {
+        deallocate3DContinuousArray(*f.first, *f.second);
+        *f.first = nullptr;
+        *f.second = nullptr;
+    }
+}
+
+vvm_host_view vvm::host_view() {
+    vvm_host_view v{};
+    v.zeta = zetacont, v.zetam = zetamcont, v.zetap = zetapcont;
+    v.th = thcont, v.thm = thmcont, v.thp = thpcont;
+    v.qv = qvcont, v.qvm = qvmcont, v.qvp = qvpcont;
+    v.qc = qccont, v.qcm = qcmcont, v.qcp = qcpcont;
+    v.qr = qrcont, v.qrm = qrmcont, v.qrp = qrpcont;
+    v.u = ucont, v.w = wcont, v.RKM = RKMcont, v.RKH = RKHcont, v.init_th_forcing = init_th_forcingcont;
+    v.evaporation = evaporationcont, v.accretion = accretioncont, v.autoconversion = autoconversioncont;
+    v.condensation = condensationcont;
+    v.dth_advect = dth_advectcont, v.dth_buoyancy = dth_buoyancycont, v.dzeta_advect = dzeta_advectcont;
+    v.dqv_advect = dqv_advectcont, v.dqc_advect = dqc_advectcont, v.dqr_advect = dqr_advectcont;
+    v.dqr_VT = dqr_VTcont;
+    v.precip = precip, v.xi = xi, v.uxi = uxi;
+    v.rhou = rhou, v.rhow = rhow, v.pib = pib, v.pb = pb, v.thb = thb, v.thbm = thbm, v.thb_init = thb_init;
+    v.thb_zeta = thb_zeta, v.thvb = thvb, v.thvbm = thvbm, v.qvb = qvb, v.qvb0 = qvb0, v.qvsb = qvsb;
+    v.z = z, v.z_zeta = z_zeta, v.lambda2 = lambda2, v.Q1LS = Q1LS, v.Q2LS = Q2LS;
+    v.ubarTop = ubarTop, v.ubarTopp = ubarTopp, v.ubarTopm = ubarTopm;
+    v.dubarTop_advect[0] = dubarTop_advect[0], v.dubarTop_advect[1] = dubarTop_advect[1];
+    v.step = step;
+    v.status_for_adding_forcing = status_for_adding_forcing ? 1 : 0;
+    return v;
+}
+
+void vvm::sync_to_device() {
+    if (!dev_stale) return;
+    if (host_stale) {
+        std::fprintf(stderr, "vvm: host and device copies both modified\n");
+        std::exit(1);
+    }
+    vvm_host_view v = host_view();
+    check(*this, vvm_dev_upload(dev, &v), "vvm_dev_upload");
+    dev_stale = false;
+}
+
+void vvm::sync_to_host() {
+    if (!host_stale) return;
+    vvm_host_view v = host_view();
+    check(*this, vvm_dev_download(dev, &v), "vvm_dev_download");
+    ubarTop = v.ubarTop, ubarTopp = v.ubarTopp, ubarTopm = v.ubarTopm;
+    dubarTop_advect[0] = v.dubarTop_advect[0], dubarTop_advect[1] = v.dubarTop_advect[1];
+    step = v.step;
+    status_for_adding_forcing = v.status_for_adding_forcing != 0;
+    host_stale = false;
+}
+
+// src/Boundary.cpp:3-31
+void vvm::BoundaryProcess1D_center(double *var, int nz) {
+    var[0] = var[1];
+    var[nz - 1] = var[nz - 2];
+}
+
+void vvm::BoundaryProcess2D_center(double **var, int nx, int nz) {
+    for (int k = 1; k <= nz - 2; k++) {
+        var[0][k] = var[nx - 2][k];
+        var[nx - 1][k] = var[1][k];
+    }
+    for (int i = 0; i < nx; i++) {
+        var[i][0] = var[i][1];
+        var[i][nz - 1] = var[i][nz - 2];
+    }
+}
+
+void vvm::BoundaryProcess2D_westdown(double **var, int nx, int nz) {
+    for (int k = 1; k <= nz - 2; k++) {
+        var[0][k] = var[nx - 2][k];
+        var[nx - 1][k] = var[1][k];
+    }
+    for (int i = 0; i < nx; i++) var[i][0] = var[i][1] = var[i][nz - 1] = 0.;
+}
+
+void vvm::BoundaryProcess2D_all(vvm &model) { run_op(model, VVM_OP_BC_ALL, "BoundaryProcess2D_all"); }
+
+void vvm::AddForcing(vvm &model) { run_op(model, VVM_OP_FORCING_BC, "AddForcing"); }
+
+void vvm::PoissonSolver::InitPoissonMatrix(vvm &model) {
+    if (model.dev) return;
+    int rc = vvm_dev_create(&model.dev_config, model.gpu_id, &model.dev);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: vvm_dev_create failed (%d)%s%s\n", rc, model.dev ? ": " : "",
+                     model.dev ? vvm_dev_last_error(model.dev) : "");
+        std::exit(1);
+    }
+    model.dev_stale = true;
+    model.host_stale = false;
+}
+
+void vvm::PoissonSolver::cal_w(vvm &model, int, int, int) { run_op(model, VVM_OP_CAL_W, "cal_w"); }
+void vvm::PoissonSolver::cal_u(vvm &model) { run_op(model, VVM_OP_CAL_U, "cal_u"); }
+void vvm::PoissonSolver::pubarTop_pt(vvm &model) { run_op(model, VVM_OP_PUBARTOP, "pubarTop_pt"); }
+void vvm::Turbulence::RKM_RKH(vvm &model) { run_op(model, VVM_OP_TURBULENCE, "RKM_RKH"); }
+void vvm::NumericalProcess::Nudge(vvm &model) { run_op(model, VVM_OP_NUDGE, "Nudge"); }
+void vvm::MicroPhysics::Kessler(vvm &model) { run_op(model, VVM_OP_MICROPHYSICS, "Kessler"); }
+void vvm::Iteration::pzeta_pt(vvm &model) { run_op(model, VVM_OP_PZETA, "pzeta_pt"); }
+void vvm::Iteration::pthermo_pt(vvm &model) { run_op(model, VVM_OP_PSCALARS, "pthermo_pt"); }
+void vvm::Iteration::nextTimeStep(vvm &model) { run_op(model, VVM_OP_NEXT, "nextTimeStep"); }
+
+void vvm::Init::InitCase(vvm &model, const vvm_case_params &p) {
+    model.sync_to_host();
+    vvm_host_view v = model.host_view();
+    int rc = vvm_init_case(&model.dev_config, &p, &v);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: vvm_init_case failed (%d)\n", rc);
+        std::exit(1);
+    }
+    model.ubarTop = v.ubarTop, model.ubarTopp = v.ubarTopp, model.ubarTopm = v.ubarTopm;
+    model.step = v.step;
+    model.dev_stale = true;
+}
+
+void vvm::Init::RandomPerturbation(vvm &model, int t, double min_range, double max_range,
+                                   double standard_deviation) {
+    model.sync_to_host();
+    vvm_host_view v = model.host_view();
+    int rc = vvm_random_perturbation(&model.dev_config, &v, t, min_range, max_range, standard_deviation);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: vvm_random_perturbation failed (%d)\n", rc);
+        std::exit(1);
+    }
+    model.dev_stale = true;
+}
+
+void vvm::Output::create_all_directory(vvm &model) {
+    if (model.outputpath.empty()) return;
+    std::string dirs[] = {model.outputpath, model.outputpath + "/state"};
+    for (const auto &d : dirs)
+        if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) {
+            std::fprintf(stderr, "vvm: cannot create %s\n", d.c_str());
+            std::exit(1);
+        }
+}
+
+// VVMSTATE record file: "VVMSTATE", int32 version 1, int32 record count,
+// then per record a 32-byte name, int32 ndim (3), int32 dims[3] and the
+// little-endian doubles ([i][k] arrays keep k fastest).
+void vvm::Output::output_state(int step, vvm &model) {
+    write_state(model.outputpath + "/state/state_" + std::to_string(step) + ".bin", model);
+}
+
+void vvm::Output::write_state(const std::string &path, vvm &model) {
+    model.sync_to_host();
+    std::vector<std::pair<std::string, std::vector<double>>> recs;
+    auto one = [&](const char *n, double x) { recs.push_back({n, {x}}); };
+    one("cfg.nx", model.nx), one("cfg.nz", model.nz), one("cfg.dx", model.dx), one("cfg.dz", model.dz);
+    one("cfg.dt", model.dt), one("cfg.GRAVITY", model.GRAVITY), one("cfg.Cp", model.Cp), one("cfg.Cv", model.Cv);
+    one("cfg.Rd", model.Rd), one("cfg.Lv", model.Lv), one("cfg.P0", model.P0), one("cfg.PSURF", model.PSURF);
+    one("cfg.addforcingtime", model.addforcingtime), one("cfg.moisture_nudge_time", model.moisture_nudge_time);
+    one("cfg.tolerance", model.tolerance), one("cfg.CASE", model.CASE);
+    struct A {
+        const char *n;
+        const double *p;
+        int d0, d1, d2;
+    };
+    const int nx = model.nx, nz = model.nz;
+    const A arrs[] = {
+        {"zeta", model.zetacont, nx, nz, 1}, {"zetam", model.zetamcont, nx, nz, 1},
+        {"zetap", model.zetapcont, nx, nz, 1}, {"th", model.thcont, nx, nz, 1}, {"thm", model.thmcont, nx, nz, 1},
+        {"thp", model.thpcont, nx, nz, 1}, {"qv", model.qvcont, nx, nz, 1}, {"qvm", model.qvmcont, nx, nz, 1},
+        {"qvp", model.qvpcont, nx, nz, 1}, {"qc", model.qccont, nx, nz, 1}, {"qcm", model.qcmcont, nx, nz, 1},
+        {"qcp", model.qcpcont, nx, nz, 1}, {"qr", model.qrcont, nx, nz, 1}, {"qrm", model.qrmcont, nx, nz, 1},
+        {"qrp", model.qrpcont, nx, nz, 1}, {"u", model.ucont, nx, nz, 1}, {"w", model.wcont, nx, nz, 1},
+        {"RKM", model.RKMcont, nx, nz, 1}, {"RKH", model.RKHcont, nx, nz, 1},
+        {"init_th_forcing", model.init_th_forcingcont, nx, nz, 1},
+        {"evaporation", model.evaporationcont, nx, nz, 1}, {"accretion", model.accretioncont, nx, nz, 1},
+        {"autoconversion", model.autoconversioncont, nx, nz, 1},
+        {"condensation", model.condensationcont, nx, nz, 1}, {"dth_advect", model.dth_advectcont, nx, nz, 2},
+        {"dth_buoyancy", model.dth_buoyancycont, nx, nz, 2}, {"dzeta_advect", model.dzeta_advectcont, nx, nz, 2},
+        {"dqv_advect", model.dqv_advectcont, nx, nz, 2}, {"dqc_advect", model.dqc_advectcont, nx, nz, 2},
+        {"dqr_advect", model.dqr_advectcont, nx, nz, 2}, {"dqr_VT", model.dqr_VTcont, nx, nz, 2},
+        {"precip", model.precip, nx, 1, 1}, {"xi", model.xi, nx, 1, 1}, {"uxi", model.uxi, nx, 1, 1},
+        {"rhou", model.rhou, nz, 1, 1}, {"rhow", model.rhow, nz, 1, 1}, {"pib", model.pib, nz, 1, 1},
+        {"pb", model.pb, nz, 1, 1}, {"thb", model.thb, nz, 1, 1}, {"thbm", model.thbm, nz, 1, 1},
+        {"thb_init", model.thb_init, nz, 1, 1}, {"thb_zeta", model.thb_zeta, nz, 1, 1},
+        {"thvb", model.thvb, nz, 1, 1}, {"thvbm", model.thvbm, nz, 1, 1}, {"qvb", model.qvb, nz, 1, 1},
+        {"qvb0", model.qvb0, nz, 1, 1}, {"qvsb", model.qvsb, nz, 1, 1}, {"z", model.z, nz, 1, 1},
+        {"z_zeta", model.z_zeta, nz, 1, 1}, {"lambda2", model.lambda2, nz, 1, 1}, {"Q1LS", model.Q1LS, nz, 1, 1},
+        {"Q2LS", model.Q2LS, nz, 1, 1}};
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        std::fprintf(stderr, "vvm: cannot write %s\n", path.c_str());
+        std::exit(1);
+    }
+    std::vector<std::pair<std::string, std::vector<double>>> tail;
+    tail.push_back({"ubarTop", {model.ubarTop}});
+    tail.push_back({"ubarTopp", {model.ubarTopp}});
+    tail.push_back({"ubarTopm", {model.ubarTopm}});
+    tail.push_back({"dubarTop_advect0", {model.dubarTop_advect[0]}});
+    tail.push_back({"dubarTop_advect1", {model.dubarTop_advect[1]}});
+    tail.push_back({"step", {double(model.step)}});
+    tail.push_back({"status_for_adding_forcing", {model.status_for_adding_forcing ? 1.0 : 0.0}});
+    int32_t head[2] = {1, int32_t(recs.size() + sizeof(arrs) / sizeof(arrs[0]) + tail.size())};
+    std::fwrite("VVMSTATE", 1, 8, f);
+    std::fwrite(head, sizeof(int32_t), 2, f);
+    auto rec = [&](const std::string &n, const double *p, int d0, int d1, int d2) {
+        char name[32] = {0};
+        std::strncpy(name, n.c_str(), 31);
+        int32_t dims[4] = {3, d0, d1, d2};
+        std::fwrite(name, 1, 32, f);
+        std::fwrite(dims, sizeof(int32_t), 4, f);
+        std::fwrite(p, sizeof(double), size_t(d0) * d1 * d2, f);
+    };
+    for (auto &r : recs) rec(r.first, r.second.data(), 1, 1, 1);
+    for (auto &a : arrs) rec(a.n, a.p, a.d0, a.d1, a.d2);
+    for (auto &r : tail) rec(r.first, r.second.data(), 1, 1, 1);
+    if (std::fclose(f) != 0) {
+        std::fprintf(stderr, "vvm: write error on %s\n", path.c_str());
+        std::exit(1);
+    }
+}
+
+// src/Iteration.cpp:208-438.  The device runs the loop body; the host
+// takes the state back only on output steps (src/Iteration.cpp:231) and at
+// the end.  Steps between two output steps go to the device in one call;
+// t_all holds the mean wall time per step of that call and the phase
+// timers the device time of its last step.
+void vvm::Iteration::TimeMarching(vvm &model) {
+    model.sync_to_host();
+    model.step = 0;
+    model.dev_stale = true;
+    const int nmax = int(model.TIMEEND / model.dt);
+    ensure_device(model);
+    while (model.step < nmax) {
+        if (model.OUTPUTSTEP > 0 && is_output_step(model, model.step) && !model.outputpath.empty())
+            vvm::Output::output_state(model.step, model);
+        int next = model.step + 1;
+        while (next < nmax && !(model.OUTPUTSTEP > 0 && is_output_step(model, next))) next++;
+        const int n = next - model.step;
+        vvm_step_stats st{};
+        int rc = vvm_dev_step(model.dev, n, &st);
+        check(model, rc, "vvm_dev_step");
+        if (st.not_converged > 0)
+            std::printf("The Poisson solver did not converge in %d of %d steps (last residual %.3e)\n",
+                        st.not_converged, n, st.rel_residual_last);
+        model.solve_iters_total += st.solve_iters_total;
+        model.host_stale = true;
+        const int T = model.TIMEROUTPUTSIZE > 0 ? model.TIMEROUTPUTSIZE : 1;
+        for (int s = model.step; s < next; s++) {
+            if (model.print_steps) std::cout << s << "\n";
+            const int slot = s % T;
+            model.t_all[slot] = st.t_all;
+            model.t_advection[slot] = st.t_advection;
+            model.t_poisson[slot] = st.t_poisson;
+            model.t_microphysics[slot] = st.t_microphysics;
+            model.t_diffusion[slot] = st.t_diffusion;
+        }
+        model.step = next;
+    }
+    if (model.print_steps) std::cout.flush();
+    model.sync_to_host();
+}

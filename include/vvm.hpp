@@ -1,0 +1,226 @@
+// vvm.hpp -- C++ state and config API of the 2D vector vorticity model,
+// with the timestep running on an MI355X through the C ABI of vvm_dev.h.
+//
+// The class and member names follow the reference's src/Declare.hpp so that
+// code written against `Config_VVM`, `vvm` and `vvm::Iteration::TimeMarching`
+// compiles against this header unchanged for the hot path:
+//   - Config_VVM keeps the reference constructor (src/Declare.hpp:29-38);
+//   - vvm keeps the host arrays `[i][k]` (k fastest) with the same names,
+//     the contiguous `<name>cont` blocks (src/Declare.hpp:400-407) and the
+//     AB2 `***` histories [i][k][2] (src/Declare.hpp:417-427);
+//   - the static operator classes (src/Declare.hpp:707-941) keep their
+//     names; each runs on the device and leaves the host arrays updated.
+//
+// Ownership is the reference's: vvm owns every host array.  The device
+// handle (`dev`) belongs to vvm and is created by
+// PoissonSolver::InitPoissonMatrix (or lazily by the first operator);
+// device memory is released with the model.
+//
+// Errors: a device error prints the C layer's message and calls exit(1),
+// as the reference's GPU back-end does (src/Declare.hpp:767-772).  A solve
+// that misses its stop rule prints a warning and the step continues
+// (src/PoissonSolver.cpp:463-467).
+#pragma once
+
+#include <string>
+
+#include "vvm_dev.h"
+
+class Config_VVM {
+   public:
+    Config_VVM(double dt, double dx, double dz, int XRANGE, int ZRANGE, double TIMEEND, int TIMEROUTPUTSIZE,
+               std::string outputpath, int OUTPUTSTEP, double Kx, double Kz, double TIMETS, double tolerance,
+               double GRAVITY, double Cp, double Cv, double Rd, double Lv, double P0, double PSURF,
+               double addforcingtime, int CASE, double mositure_nudge_time, int year, int month, int day, double hour,
+               double minute, double second, double lon, double lat);
+    ~Config_VVM() {}
+
+    double dt, dx, dz;
+    int XRANGE, ZRANGE;  // domain size in m, including one ghost cell each side
+    double TIMEEND;
+    int TIMEROUTPUTSIZE;
+    std::string outputpath;
+    int OUTPUTSTEP;
+    double Kx, Kz, TIMETS, tolerance;
+    double GRAVITY, Cp, Cv, Rd, Lv, P0, PSURF;
+    double addforcingtime;
+    int CASE;
+    double mositure_nudge_time;
+    int year, month, day;
+    double hour, minute, second, lon, lat;
+
+    // Device back-end options (no counterpart in the reference, which picks
+    // its back-end at compile time).  Defaults reproduce the reference path:
+    // FP64 solve from a zero guess, iterated to round-off.
+    int gpu_id = 0;
+    int solver_mode = VVM_SOLVER_FP64;
+    int guess_mode = VVM_GUESS_ZERO;
+    int stop_mode = VVM_STOP_PARITY;
+    int max_iter = 2000;
+    double tol_rel = 1e-10;
+    int tropical_forcing = 0;
+};
+
+class vvm {
+   public:
+    explicit vvm(const Config_VVM &config);
+    ~vvm();
+    vvm(const vvm &) = delete;
+    vvm &operator=(const vvm &) = delete;
+
+    static double **allocate2DContinuousArray(int rows, int cols, double *&contMemory);
+    static void deallocate2DContinuousArray(double **array, double *contMemory);
+    double ***allocate3DContinuousArray(int dim1, int dim2, int dim3, double *&contMemory);
+    void deallocate3DContinuousArray(double ***array, double *contMemory);
+
+    // grid and constants (src/Declare.hpp:431-480)
+    double rdx = 0, r2dx = 0, rdz = 0, r2dz = 0, rdx2 = 0, rdz2 = 0;
+    int nx = 0, nz = 0;
+    double dt = 0, d2t = 0, dx = 0, dz = 0;
+    int XRANGE = 0, ZRANGE = 0;
+    double TIMEEND = 0;
+    int TIMEROUTPUTSIZE = 0;
+    std::string outputpath;
+    int OUTPUTSTEP = 0;
+    double Kx = 0, Kz = 0, TIMETS = 0, tolerance = 0;
+    double GRAVITY = 0, Cp = 0, Cv = 0, Rd = 0, Lv = 0, P0 = 0, PSURF = 0;
+    double addforcingtime = 0;
+    int CASE = 0;
+    int year = 2025, month = 3, day = 20;
+    double hour = 12., minute = 0., second = 0., lon = 0., lat = 0.;
+
+    // 0-D state (src/Declare.hpp:482-487)
+    int step = 0;
+    double ubarTopp = 0, ubarTop = 0, ubarTopm = 0;
+    double moisture_nudge_time = 0.;
+    double dubarTop_advect[2] = {0, 0};
+
+    // 1-D profiles, nz long (src/Declare.hpp:490-509)
+    double *thb = nullptr, *thb_init = nullptr, *thbm = nullptr, *thb_zeta = nullptr;
+    double *rhou = nullptr, *rhow = nullptr, *pib = nullptr, *qvb = nullptr, *qvb0 = nullptr, *qvsb = nullptr;
+    double *pb = nullptr, *thvb = nullptr, *thvbm = nullptr, *z = nullptr, *z_zeta = nullptr, *lambda2 = nullptr;
+    double *Q1LS = nullptr, *Q2LS = nullptr;
+    // nx-long rows
+    double *xi = nullptr, *uxi = nullptr, *precip = nullptr;
+
+    // 2-D fields [i][k] (src/Declare.hpp:546-648)
+    double **zetap = nullptr, **zeta = nullptr, **zetam = nullptr;
+    double **thp = nullptr, **th = nullptr, **thm = nullptr;
+    double **qvp = nullptr, **qv = nullptr, **qvm = nullptr;
+    double **qcp = nullptr, **qc = nullptr, **qcm = nullptr;
+    double **qrp = nullptr, **qr = nullptr, **qrm = nullptr;
+    double **u = nullptr, **w = nullptr, **RKM = nullptr, **RKH = nullptr;
+    double **evaporation = nullptr, **accretion = nullptr, **autoconversion = nullptr, **condensation = nullptr;
+    double **init_th_forcing = nullptr;
+    double *zetapcont = nullptr, *zetacont = nullptr, *zetamcont = nullptr;
+    double *thpcont = nullptr, *thcont = nullptr, *thmcont = nullptr;
+    double *qvpcont = nullptr, *qvcont = nullptr, *qvmcont = nullptr;
+    double *qcpcont = nullptr, *qccont = nullptr, *qcmcont = nullptr;
+    double *qrpcont = nullptr, *qrcont = nullptr, *qrmcont = nullptr;
+    double *ucont = nullptr, *wcont = nullptr, *RKMcont = nullptr, *RKHcont = nullptr;
+    double *evaporationcont = nullptr, *accretioncont = nullptr, *autoconversioncont = nullptr,
+           *condensationcont = nullptr;
+    double *init_th_forcingcont = nullptr;
+    bool status_for_adding_forcing = true;
+
+    // AB2 histories [i][k][2] (src/Declare.hpp:652-692)
+    double ***dth_advect = nullptr, ***dth_buoyancy = nullptr, ***dzeta_advect = nullptr;
+    double ***dqv_advect = nullptr, ***dqc_advect = nullptr, ***dqr_advect = nullptr, ***dqr_VT = nullptr;
+    double *dth_advectcont = nullptr, *dth_buoyancycont = nullptr, *dzeta_advectcont = nullptr;
+    double *dqv_advectcont = nullptr, *dqc_advectcont = nullptr, *dqr_advectcont = nullptr, *dqr_VTcont = nullptr;
+
+    // per-step phase timers, TIMEROUTPUTSIZE long (src/Declare.hpp:695-699)
+    double *t_advection = nullptr, *t_poisson = nullptr, *t_diffusion = nullptr, *t_microphysics = nullptr,
+           *t_all = nullptr;
+
+    // Device back-end.  `dev` owns the device copy of the state;
+    // `host_stale` is set while the device holds steps the host arrays
+    // have not seen, `dev_stale` while host edits have not been uploaded.
+    vvm_dev_t *dev = nullptr;
+    vvm_dev_config dev_config{};
+    int gpu_id = 0;
+    bool host_stale = false;
+    bool dev_stale = true;
+    bool print_steps = true;           // `std::cout << step` each step, as the reference
+    long long solve_iters_total = 0;   // PCG iterations over the run
+    vvm_host_view host_view();          // pointers into this model's arrays
+    void sync_to_device();              // upload when dev_stale
+    void sync_to_host();                // download when host_stale
+
+    // Host-side halo rules of src/Boundary.cpp:3-33 (periodic x; centre
+    // rows copied, west/down rows zeroed), for arrays the caller edits.
+    static void BoundaryProcess1D_center(double *var, int nz);
+    static void BoundaryProcess2D_center(double **var, int nx, int nz);
+    static void BoundaryProcess2D_westdown(double **var, int nx, int nz);
+    // src/Boundary.cpp:34-52, on the device state.
+    static void BoundaryProcess2D_all(vvm &model);
+
+    // Forcing of src/AddForcing.cpp:6-28 followed by BoundaryProcess2D_all,
+    // as called at src/Iteration.cpp:283-285.
+    static void AddForcing(vvm &model);
+
+    class PoissonSolver {
+       public:
+        // Creates the device back-end (matrix-free: the operator is held as
+        // its 1-D coefficient profiles) and uploads the state.
+        static void InitPoissonMatrix(vvm &model);
+        static void cal_w(vvm &model, int p = 0, int i = 0, int j = 0);
+        static void cal_u(vvm &model);
+        static void pubarTop_pt(vvm &model);
+    };
+
+    class Turbulence {
+       public:
+        // K_M, K_H (src/Turbulent.cpp:5-66) and the in-place diffusion of
+        // zeta, th, qv, qc, qr (Mparam / Hparam, src/Turbulent.cpp:68-114).
+        static void RKM_RKH(vvm &model);
+    };
+
+    class NumericalProcess {
+       public:
+        // Nudge_theta, Nudge_zeta (unless CASE == 2) and Nudge_qv
+        // (src/NumericalProcess.cpp:60-96, src/Iteration.cpp:415-417).
+        static void Nudge(vvm &model);
+    };
+
+    class MicroPhysics {
+       public:
+        // NegativeValueProcess on qvp/qcp/qrp, condensation, autoconversion,
+        // accretion, evaporation and the halos (src/Iteration.cpp:302-312,404)
+        // in one device pass.
+        static void Kessler(vvm &model);
+    };
+
+    class Init {
+       public:
+        // Base state and initial fields of the idealised cases (warm bubble
+        // over the exponential-theta sounding); see vvm_init_case.
+        static void InitCase(vvm &model, const vvm_case_params &p);
+        static void RandomPerturbation(vvm &model, int t, double min_range = -0.25, double max_range = 0.25,
+                                       double standard_deviation = 1.);
+    };
+
+    class Output {
+       public:
+        static void create_all_directory(vvm &model);
+        // The model state at `step` as a VVMSTATE record file
+        // <outputpath>/state/state_<step>.bin.
+        static void output_state(int step, vvm &model);
+        // The same record file at an explicit path.
+        static void write_state(const std::string &path, vvm &model);
+    };
+
+    class Iteration {
+       public:
+        static void pzeta_pt(vvm &model);
+        // pth_pt, pqv_pt, pqc_pt and pqr_pt (src/Iteration.cpp:36-40,90-124)
+        // act on independent fields and run as one device pass.
+        static void pthermo_pt(vvm &model);
+        static void TimeMarching(vvm &model);
+        static void nextTimeStep(vvm &model);
+    };
+
+   private:
+    void allocateMemory();
+    void deallocateMemory();
+};
