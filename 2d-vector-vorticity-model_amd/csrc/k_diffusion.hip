@@ -25,6 +25,10 @@
 // instead of a hang.  Values agree with the sequential sweep to rounding.
 #pragma clang fp contract(off)
 
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "vvm_internal.h"
 
 namespace vvmd {
@@ -35,6 +39,12 @@ constexpr int kMaxWaves = 16;
 constexpr int kSpinLimit = 1 << 24;
 constexpr unsigned long long kEmpty = 0x7FF4DEADBEEF0001ull;  // a NaN payload arithmetic never makes
 constexpr int BX = 64, BY = 4;
+constexpr int kMinStripWaves = 2;   // narrowest strip: 2 compute waves (128 columns)
+constexpr int kDefStripWaves = 4;   // default strip: 256 columns
+__host__ __device__ inline int strip_waves(const StepPtrs &S) { return S.dstrip_cw > 0 ? S.dstrip_cw : kDefStripWaves; }
+__host__ __device__ inline int strips_of(const Grid &G, int cw) { return (G.nx - 2 + cw * 64 - 1) / (cw * 64); }
+constexpr int kMaxStrips = 64;
+constexpr int kSlotSpinLimit = 1 << 18;
 
 struct Affine {
     double a, b;  // y = a x + b
@@ -47,6 +57,13 @@ __device__ inline Affine after(const Affine &g, const Affine &f) { return {g.a *
 // :105-110 for Hparam on th, qv, qc, qr).  Row k = 1 folds the (old) ghost
 // row below into c.
 __global__ void __launch_bounds__(BX *BY) k_diff_coeffs(Grid G, StepPtrs S) {
+    if (S.dslots) {  // empty carry slots for the sweep that follows
+        const int nslot = NPROG * strips_of(G, strip_waves(S)) * G.nz;
+        const int nthr = gridDim.x * gridDim.y * BX * BY;
+        for (int t = (blockIdx.y * gridDim.x + blockIdx.x) * (BX * BY) + threadIdx.y * BX + threadIdx.x; t < nslot;
+             t += nthr)
+            __hip_atomic_store(S.dslots + t, kEmpty, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
     if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
     const size_t p = ix(G, i, k);
@@ -202,6 +219,187 @@ __global__ void __launch_bounds__(kMaxWaves *kWave) k_diff_sweep(Grid G, StepPtr
              fld == F_TH, S.P.thb_init, carry, S.s + S_DIFF_ERR);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-workgroup sweep.  The columns of a field are cut into strips of
+// W = cw * 64 * E columns; strip s of field f is workgroup (s, f), and the
+// strips of a field run on different CUs as a pipeline along the rows.
+// Inside a workgroup, cw compute wavefronts run the same wave pipeline as
+// the one-workgroup sweep (LDS carries between neighbouring waves).  One
+// more wavefront, the courier, moves carries between workgroups: it fetches
+// the carries of the strip on the left (up to 64 rows per round trip) into
+// LDS for compute wave 0, and publishes the last compute wave's carries for
+// the strip on the right.  Only the courier touches the inter-workgroup
+// slots, so its long-latency loads never hold up the compute waves' own
+// memory counters.
+//
+// The slots live in uncached device memory: a relaxed agent-scope store is
+// visible to a load on any CU without cache maintenance.  Each carry is its
+// own ready flag (the kEmpty NaN payload marks an empty slot; k_diff_coeffs
+// empties the slots before every sweep).  Workgroups are numbered
+// strip-fastest, so every producer is dispatched before its consumer, and
+// all workgroups of a launch (strips x 5 fields) are resident at once.
+// Every wait has a spin limit; a courier that gives up fills the remaining
+// carries with NaN so that its compute waves finish, and sets the error flag.
+constexpr int kMaxCompute = 8;
+
+__device__ inline unsigned long long slot_load(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long lds_load(const unsigned long long *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ inline void lds_store(unsigned long long *p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Lanes 0..63 look at rows k0 + lane; returns how many rows from k0 on are
+// ready (values not kEmpty), counting rows past kend as ready.
+__device__ inline int ready_run(unsigned long long v, int k0, int kend) {
+    const int lane = threadIdx.x % kWave;
+    const unsigned long long empty = __ballot(k0 + lane <= kend && v == kEmpty);
+    const int span = min(kWave, kend - k0 + 1);
+    return empty ? min(__builtin_ctzll(empty), span) : span;
+}
+
+__device__ void courier(const unsigned long long *in, unsigned long long *out, unsigned long long *cin,
+                        const unsigned long long *clast, int nz, double *err) {
+    const int lane = threadIdx.x % kWave;
+    const int kend = nz - 2;
+    int kin = in ? 1 : kend + 1, kout = out ? 1 : kend + 1;
+    int idle = 0;
+    while (kin <= kend || kout <= kend) {
+        bool moved = false;
+        if (kin <= kend) {
+            const int k = kin + lane;
+            const unsigned long long v = (k <= kend) ? slot_load(in + k) : 0ull;
+            const int r = ready_run(v, kin, kend);
+            if (lane < r) lds_store(cin + k, v);
+            kin += r;
+            moved = moved || r > 0;
+        }
+        if (kout <= kend) {
+            const int k = kout + lane;
+            const unsigned long long v = (k <= kend) ? lds_load(clast + k) : 0ull;
+            const int r = ready_run(v, kout, kend);
+            if (lane < r)
+                __hip_atomic_store(out + k, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            kout += r;
+            moved = moved || r > 0;
+        }
+        if (moved) {
+            idle = 0;
+        } else if (++idle > kSlotSpinLimit) {
+            if (lane == 0) *err = 1.;
+            // unblock compute wave 0: the rows not fetched become NaN
+            const unsigned long long qnan = 0x7FF8000000000000ull;
+            for (int k = kin + lane; k <= kend; k += kWave) lds_store(cin + k, qnan);
+            break;
+        } else {
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+}
+
+template <int E>
+__global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid G, StepPtrs S, int nudge,
+                                                                          int nstrip) {
+    extern __shared__ unsigned long long lds[];  // carry[cw][nz], cin[nz]
+    const int cw = blockDim.x / kWave - 1;
+    const int nz = G.nz, n = G.nx - 2;
+    unsigned long long *carry = lds, *cin = lds + size_t(cw) * nz;
+    for (int t = threadIdx.x; t < (cw + 1) * nz; t += blockDim.x) lds[t] = kEmpty;
+    __syncthreads();
+    const int s = blockIdx.x, fld = blockIdx.y;
+    const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const unsigned long long *in = (s > 0) ? S.dslots + (size_t(fld) * nstrip + (s - 1)) * nz : nullptr;
+    unsigned long long *out = (s + 1 < nstrip) ? S.dslots + (size_t(fld) * nstrip + s) * nz : nullptr;
+    if (w == cw) {
+        courier(in, out, cin, carry + size_t(cw - 1) * nz, nz, S.s + S_DIFF_ERR);
+        return;
+    }
+    const bool mpar = (fld == F_ZETA);
+    const double *__restrict__ A_ = S.dcoef[mpar ? DC_AM : DC_AH];
+    const double *__restrict__ C_ = S.dcoef[DC_C0 + fld];
+    const double *__restrict__ D_ = S.dcoef[mpar ? DC_DM : DC_DH];
+    double *f = S.fut[fld];
+    const double *rate = mpar ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
+    const bool is_theta = (fld == F_TH);
+    const int W = cw * kWave * E;
+    const int strip0 = 1 + s * W + w * kWave * E;       // first column of this wave
+    const int strip1 = min(strip0 + kWave * E - 1, n);  // last column of this wave
+    const int i0 = strip0 + lane * E;
+    // waves past the last column of a partial strip have no work; every
+    // wave after them is past it too, so nobody waits for their carries
+    if (strip0 > n) return;
+    // where this wave's carry comes from: the wave on its left, the
+    // courier (wave 0 of strips s > 0), or the old ghost column (strip 0)
+    const unsigned long long *src = (w > 0) ? carry + size_t(w - 1) * nz : (in ? cin : nullptr);
+    unsigned long long *dst = carry + size_t(w) * nz;
+    const bool publish = (w < cw - 1) || (out != nullptr);
+
+    double ba[PF][E], bc[PF][E], bd[PF][E], ynew[E];
+#pragma unroll
+    for (int j = 0; j < E; j++) ynew[j] = 0.;
+#pragma unroll
+    for (int q = 0; q < PF; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
+    bool gave_up = false;
+    for (int k0 = 1; k0 <= nz - 2; k0 += PF) {
+#pragma unroll
+        for (int q = 0; q < PF; q++) {
+            const int k = k0 + q;
+            if (k > nz - 2) break;
+            double a[E], e[E];
+            Affine m{1., 0.};
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                a[j] = ba[q][j];
+                e[j] = bc[q][j] + bd[q][j] * ynew[j];
+                if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
+            }
+            load_row<E>(G, A_, C_, D_, i0, n, k + PF, nz, ba[q], bc[q], bd[q]);
+            Affine inc = m;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const double pa = __shfl_up(inc.a, off, kWave), pb = __shfl_up(inc.b, off, kWave);
+                if (lane >= off) inc = after(inc, Affine{pa, pb});
+            }
+            const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
+            double y0;
+            if (!src) {
+                y0 = f[ix(G, 0, k)];  // ghost column, old as in the reference
+            } else {
+                unsigned long long bits = kEmpty;
+                if (lane == 0) {
+                    int spins = 0;
+                    while ((bits = lds_load(src + k)) == kEmpty && !gave_up) {
+                        if (++spins > kSpinLimit) {
+                            S.s[S_DIFF_ERR] = 1.;
+                            gave_up = true;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                y0 = __shfl(__longlong_as_double(static_cast<long long>(bits)), 0, kWave);
+            }
+            double y = (lane == 0) ? y0 : ea * y0 + eb;
+            const double r = rate ? rate[k] : 0.;
+#pragma unroll
+            for (int j = 0; j < E; j++) {
+                const int i = i0 + j;
+                if (i > n) continue;
+                y = a[j] * y + e[j];
+                ynew[j] = y;
+                double o = y;
+                if (r != 0.) o = is_theta ? y - r * (y - S.P.thb_init[k]) : y - r * y;
+                f[ix(G, i, k)] = o;
+                if (i == strip1 && publish)
+                    lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
+            }
+        }
+    }
+}
+
+
 template <int E>
 void launch_e(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
     const int n = G.nx - 2;
@@ -212,14 +410,65 @@ void launch_e(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
 
 }  // namespace
 
+bool diffusion_strip_waves_ok(int cw) { return cw >= kMinStripWaves && cw <= kMaxCompute; }
+
 // nx - 2 <= 16 waves * 64 lanes * 8 columns, and the carry rows fit in LDS.
 bool diffusion_supported(const Grid &G) {
     return G.nx - 2 <= kMaxWaves * kWave * 8 && size_t(kMaxWaves) * G.nz * sizeof(double) <= 60 * 1024;
 }
 
+// Carry-slot regions of the multi-workgroup sweep, one per device handle.
+// Uncached memory is never returned to the runtime: a region of a
+// destroyed handle goes back to this pool for the next handle.
+namespace {
+struct SlotRegion {
+    int device;
+    size_t bytes;
+    unsigned long long *p;
+    bool busy;
+};
+std::mutex g_slot_mu;
+std::vector<SlotRegion> g_slots;
+}  // namespace
+
+unsigned long long *diffusion_slots(const Grid &G, int device) {
+    // room for the narrowest strips the tuner may pick
+    const int ns = strips_of(G, kMinStripWaves);
+    if (ns < 2 || ns > kMaxStrips) return nullptr;
+    const size_t need = size_t(NPROG) * ns * G.nz * sizeof(unsigned long long);
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    for (SlotRegion &r : g_slots)
+        if (!r.busy && r.device == device && r.bytes >= need) {
+            r.busy = true;
+            return r.p;
+        }
+    const size_t bytes = std::max(need, size_t(64) << 10);
+    void *p = nullptr;
+    if (hipExtMallocWithFlags(&p, bytes, hipDeviceMallocUncached) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_slots.push_back({device, bytes, static_cast<unsigned long long *>(p), true});
+    return g_slots.back().p;
+}
+
+void diffusion_slots_release(unsigned long long *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_slot_mu);
+    for (SlotRegion &r : g_slots)
+        if (r.p == p) r.busy = false;
+}
+
 void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
     hipLaunchKernelGGL(k_diff_coeffs, dim3((G.nx + BX - 1) / BX, (G.nz + BY - 1) / BY), dim3(BX, BY), 0, st, G, S);
     const int n = G.nx - 2;
+    if (S.dslots) {
+        const int cw = strip_waves(S);  // compute waves of one column per lane
+        const int ns = strips_of(G, cw);
+        const size_t lds = size_t(cw + 1) * G.nz * sizeof(unsigned long long);
+        hipLaunchKernelGGL(k_diff_strips<1>, dim3(ns, NPROG), dim3((cw + 1) * kWave), lds, st, G, S, nudge, ns);
+        return;
+    }
     if (n <= kMaxWaves * kWave) launch_e<1>(G, S, nudge, st);
     else if (n <= 2 * kMaxWaves * kWave) launch_e<2>(G, S, nudge, st);
     else if (n <= 4 * kMaxWaves * kWave) launch_e<4>(G, S, nudge, st);

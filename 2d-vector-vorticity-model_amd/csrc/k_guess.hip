@@ -1,0 +1,147 @@
+// k_guess.hip -- the learned initial guess of the cal_w solve and the
+// recording of training samples.
+//
+// Features on the unknown grid (rows 1..M, N periodic columns), each scaled
+// by its RMS over the grid:
+//   0: r = f - L xe   (residual of the extrapolated guess xe = 2 x1 - x2)
+//   1: d = x1 - x2    (last solution increment)
+//   2: xe
+// The network predicts (x - xe) / s_d with s_d = RMS(d); the guess is
+// xe + s_d * prediction.  Everything before the network is FP64; the
+// features enter the network as bf16.
+#pragma clang fp contract(off)
+
+#include <cmath>
+
+#include "poisson.h"
+
+namespace vvmd {
+
+namespace {
+
+constexpr int GX = 64, GY = 4;
+inline dim3 ggrid(int N, int M) { return dim3((N + GX - 1) / GX, (M + GY - 1) / GY); }
+
+__device__ inline double xe_at(const double *x1, const double *x2, size_t a) { return 2. * x1[a] - x2[a]; }
+
+__global__ void __launch_bounds__(GX *GY) k_guess_features(int N, int M, const double *__restrict__ f,
+                                                           const double *__restrict__ x1,
+                                                           const double *__restrict__ x2,
+                                                           const double *__restrict__ cx,
+                                                           const double *__restrict__ cu, float *__restrict__ feat,
+                                                           RedSlot rs) {
+    const int i = blockIdx.x * GX + threadIdx.x, j = 1 + blockIdx.y * GY + threadIdx.y;
+    double part[3] = {0., 0., 0.};
+    if (i < N && j <= M) {
+        const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
+        const size_t r = size_t(j) * N;
+        const double c = xe_at(x1, x2, r + i);
+        const double Lx = cx[j] * (2. * c - xe_at(x1, x2, r + im) - xe_at(x1, x2, r + ip)) +
+                          cu[j] * (c - xe_at(x1, x2, r + N + i)) + cu[j - 1] * (c - xe_at(x1, x2, r - N + i));
+        const double res = f[r + i] - Lx, d = x1[r + i] - x2[r + i];
+        const size_t plane = size_t(M) * N, q = size_t(j - 1) * N + i;
+        feat[q] = float(res);
+        feat[plane + q] = float(d);
+        feat[2 * plane + q] = float(c);
+        part[0] = res * res;
+        part[1] = d * d;
+        part[2] = c * c;
+    }
+    block_partials<3>(part, rs);
+}
+
+// sums of squares -> scales (0 for an all-zero feature)
+__device__ inline double inv_rms(double s2, double n) { return s2 > 0. ? 1. / sqrt(s2 / n) : 0.; }
+
+__global__ void k_guess_normalize(size_t plane, double npts, const double *__restrict__ gsc, float *__restrict__ feat,
+                                  float *__restrict__ dst) {
+    const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= 3 * plane) return;
+    const int c = int(t / plane);
+    dst[t] = float(double(feat[t]) * inv_rms(gsc[c], npts));
+}
+
+__global__ void __launch_bounds__(GX *GY) k_guess_apply(int N, int M, const double *__restrict__ x1,
+                                                        const double *__restrict__ x2, const float *__restrict__ delta,
+                                                        const double *__restrict__ gsc, double *__restrict__ x) {
+    const int i = blockIdx.x * GX + threadIdx.x, j = blockIdx.y * GY + threadIdx.y;
+    if (i >= N || j > M + 1) return;
+    const size_t a = size_t(j) * N + i;
+    if (j == 0 || j == M + 1) {
+        x[a] = 0.;
+        return;
+    }
+    const double sd = sqrt(gsc[1] / (double(N) * M));
+    x[a] = xe_at(x1, x2, a) + sd * double(delta[size_t(j - 1) * N + i]);
+}
+
+__global__ void __launch_bounds__(GX *GY) k_guess_target(int N, int M, const double *__restrict__ x,
+                                                         const double *__restrict__ x1,
+                                                         const double *__restrict__ x2,
+                                                         const double *__restrict__ gsc, float *__restrict__ tgt) {
+    const int i = blockIdx.x * GX + threadIdx.x, j = 1 + blockIdx.y * GY + threadIdx.y;
+    if (i >= N || j > M) return;
+    const size_t a = size_t(j) * N + i;
+    const double isd = inv_rms(gsc[1], double(N) * M);
+    tgt[size_t(j - 1) * N + i] = float((x[a] - xe_at(x1, x2, a)) * isd);
+}
+
+}  // namespace
+
+// features of the current (f, x_{t-1}, x_{t-2}), normalised, into dst [3][M][N]
+void Poisson::guess_features(float *dst, hipStream_t st) {
+    const size_t plane = size_t(M) * N;
+    if (!gfeat) {
+        hipMalloc(reinterpret_cast<void **>(&gfeat), sizeof(float) * 3 * plane);
+        hipMalloc(reinterpret_cast<void **>(&gdelta), sizeof(float) * plane);
+        hipMalloc(reinterpret_cast<void **>(&gsc), sizeof(double) * 4);
+    }
+    if (!dst) dst = gfeat;  // in place: each element reads and writes its own slot
+    const dim3 g = ggrid(N, M);
+    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
+                       pcg64.mg.lev[0].cu, gfeat, red);
+    finalize<3>(red, nblocks(g), gsc, RED_STORE, 0, st);
+    hipLaunchKernelGGL(k_guess_normalize, dim3(unsigned((3 * plane + 255) / 256)), dim3(256), 0, st, plane,
+                       double(plane), gsc, gfeat, dst);
+}
+
+bool Poisson::learned_guess(hipStream_t st) {
+    if (!net || nsolves < 2) return false;
+    guess_features(nullptr, st);
+    const NetGeo g{1, M, N, 1};
+    if (net->forward(g, gfeat, gdelta, false, st) != VVM_OK) return false;
+    hipLaunchKernelGGL(k_guess_apply, ggrid(N, M + 2), dim3(GX, GY), 0, st, N, M, xprev, xprev2, gdelta, gsc,
+                       pcg64.x);
+    return true;
+}
+
+int Poisson::collect_reserve(int cap) {
+    if (samples) hipFree(samples);
+    samples = nullptr;
+    sample_cap = sample_count = 0;
+    sample_open = false;
+    if (cap <= 0) return VVM_OK;
+    if (hipMalloc(reinterpret_cast<void **>(&samples), sizeof(float) * 4 * size_t(M) * N * cap) != hipSuccess) {
+        samples = nullptr;
+        return VVM_ERR_NOMEM;
+    }
+    sample_cap = cap;
+    return VVM_OK;
+}
+
+void Poisson::collect_features(hipStream_t st) {
+    sample_open = false;
+    if (!samples || sample_count >= sample_cap || nsolves < 2) return;
+    guess_features(samples + size_t(sample_count) * 4 * M * N, st);
+    sample_open = true;
+}
+
+void Poisson::collect_target(hipStream_t st) {
+    if (!sample_open) return;
+    float *tgt = samples + (size_t(sample_count) * 4 + 3) * M * N;
+    hipLaunchKernelGGL(k_guess_target, ggrid(N, M), dim3(GX, GY), 0, st, N, M, pcg64.x, xprev, xprev2, gsc, tgt);
+    sample_count++;
+    sample_open = false;
+}
+
+}  // namespace vvmd

@@ -286,6 +286,8 @@ __global__ void __launch_bounds__(256) k_dot(int n, const T *a, const T *b, RedS
 
 template <typename T>
 int mg_plan(MGHier<T> &H, size_t lds_budget) {
+    if (H.cws) hipFree(H.cws);  // a previous plan's global workspace
+    H.cws = nullptr;
     // smallest start level whose remaining levels fit in LDS
     H.coarse_from = H.nlev;
     for (int lc = H.nlev - 1; lc >= 0; lc--) {

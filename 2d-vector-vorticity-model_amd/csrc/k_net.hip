@@ -1,0 +1,346 @@
+// k_net.hip -- the learned initial-guess network on the device: 3x3
+// convolutions as implicit GEMMs on MFMA (bf16 operands, fp32
+// accumulation), GELU, the weight gradient, Adam.
+//
+// Tensors are pixel-major with 32 bf16 channels per pixel ([b][y][x][32],
+// 64 bytes a pixel); the network input uses channels 0..2 and keeps the
+// rest zero, and layer 6 produces channel 0 only (its other weights stay
+// zero), so every layer is the same 32 -> 32 convolution.
+//
+// Forward / backward-data kernel (k_conv): a workgroup of 4 wavefronts
+// computes a 64 x 2 pixel tile; each wavefront owns 32 pixels of one row
+// and all 32 output channels, i.e. one v_mfma_f32_32x32x16_bf16 output
+// block, and walks K = 9 taps x 32 input channels in 18 steps of 16.  The
+// (64+2) x (2+2) input tile with its halo is staged once in LDS.  The
+// B operand (weights) is pre-packed in the exact lane order of the
+// instruction, so a lane fetches its 8 weights as one 16-byte load.
+//
+// Weight-gradient kernel (k_conv_wgrad): one wavefront per tap; the GEMM
+// runs over pixels (K) with the input tile and the output gradient staged
+// channel-major in LDS; partial sums per workgroup are reduced by
+// k_wgrad_reduce.
+#include <cstdint>
+
+#include "net.h"
+
+namespace vvmd {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int TW = 64, TH = 2;                 // output tile (columns x rows)
+constexpr int HW = TW + 2, HH = TH + 2;        // haloed tile
+constexpr int PXS = 40;                        // LDS pixel stride in bf16 (80 bytes)
+constexpr float GC = 0.7978845608028654f;      // sqrt(2/pi)
+constexpr float GA = 0.044715f;
+
+__device__ inline float gelu(float z) { return 0.5f * z * (1.f + tanhf(GC * (z + GA * z * z * z))); }
+__device__ inline float gelu_d(float z) {
+    const float t = tanhf(GC * (z + GA * z * z * z));
+    return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * GC * (1.f + 3.f * GA * z * z);
+}
+
+__device__ inline __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
+__device__ inline float from_bf16(__bf16 x) { return static_cast<float>(x); }
+
+// global pixel index of haloed tile position (ry, rx) or -1 for padding
+__device__ inline long halo_src(const NetGeo &g, int b, int y0, int x0, int ry, int rx) {
+    const int y = y0 + ry - 1;
+    int x = x0 + rx - 1;
+    if (y < 0 || y >= g.h) return -1;
+    if (x < 0 || x >= g.w) {
+        if (!g.periodic) return -1;
+        x = (x + g.w) % g.w;
+    }
+    return (long(b) * g.h + y) * g.w + x;
+}
+
+// MODE 0: hidden layer (bias, GELU; writes A and Z), 1: last layer (bias;
+// channel 0 to fp32), 2: backward data (times gelu'(Z of the layer below);
+// writes that layer's pre-activation gradient).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict__ in,
+                                              const bf16x8 *__restrict__ wpk, const float *__restrict__ bias,
+                                              __bf16 *__restrict__ outA, __bf16 *__restrict__ outZ,
+                                              float *__restrict__ outF, const __bf16 *__restrict__ zbelow) {
+    __shared__ __attribute__((aligned(16))) __bf16 tile[HH * HW * PXS];
+    const int b = blockIdx.z, y0 = blockIdx.y * TH, x0 = blockIdx.x * TW;
+    // stage the haloed tile: 16-byte chunks, 4 per pixel
+    for (int c = threadIdx.x; c < HH * HW * 4; c += blockDim.x) {
+        const int px = c >> 2, part = c & 3;
+        const int ry = px / HW, rx = px % HW;
+        const long src = halo_src(g, b, y0, x0, ry, rx);
+        bf16x8 v = {};
+        if (src >= 0) v = *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8);
+        *reinterpret_cast<bf16x8 *>(tile + px * PXS + part * 8) = v;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ty = wv >> 1, cb = (wv & 1) * 32;
+    const int i = lane & 31, gk = lane >> 5;
+    f32x16 acc = {};
+#pragma unroll
+    for (int s = 0; s < 18; s++) {
+        const int t = s >> 1, hh = s & 1;
+        const int dy = t / 3 - 1, dx = t % 3 - 1;
+        const int px = (ty + 1 + dy) * HW + (cb + i + 1 + dx);
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
+        const bf16x8 w = wpk[s * 64 + lane];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc, 0, 0, 0);
+    }
+    // D[m][n]: n = lane % 32 (output channel), m = 8 (r / 4) + 4 (lane / 32) + r % 4 (pixel)
+    const int co = lane & 31;
+    const int y = y0 + ty;
+    if (y >= g.h) return;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int m = 8 * (r >> 2) + 4 * gk + (r & 3);
+        const int x = x0 + cb + m;
+        if (x >= g.w) continue;
+        const long p = (long(b) * g.h + y) * g.w + x;
+        if (MODE == 0) {
+            const float z = acc[r] + bias[co];
+            outZ[p * NET_C + co] = to_bf16(z);
+            outA[p * NET_C + co] = to_bf16(gelu(z));
+        } else if (MODE == 1) {
+            if (co == 0) outF[p] = acc[r] + bias[0];
+        } else {
+            const float dz = acc[r] * gelu_d(from_bf16(zbelow[p * NET_C + co]));
+            outA[p * NET_C + co] = to_bf16(dz);
+        }
+    }
+}
+
+// Weight gradient of one layer: dW[co][ci][t] = sum_p dZ[p][co] X[p + off_t][ci]
+// and db[co] = sum_p dZ[p][co].  Workgroup = 9 wavefronts (one per tap),
+// grid-stride over 64 x 2 tiles; per-workgroup partials [blk][9][32 ci][32 co]
+// then 32 bias sums.
+__global__ void __launch_bounds__(576) k_conv_wgrad(NetGeo g, const __bf16 *__restrict__ x,
+                                                    const __bf16 *__restrict__ dz, float *__restrict__ partial) {
+    __shared__ __attribute__((aligned(16))) __bf16 xt[NET_C][HH * HW + 8];  // channel-major haloed input
+    __shared__ __attribute__((aligned(16))) __bf16 dzt[NET_C][TH * TW];     // channel-major output gradient
+    const int lane = threadIdx.x & 63, t = threadIdx.x >> 6;
+    const int dy = t / 3 - 1, dx = t % 3 - 1;
+    const int gk = lane >> 5, ch = lane & 31;
+    const int tx_n = (g.w + TW - 1) / TW, ty_n = (g.h + TH - 1) / TH;
+    const long ntile = long(tx_n) * ty_n * g.nb;
+    f32x16 acc = {};
+    float bsum = 0.f;
+    for (long tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+        const int b = int(tile / (long(tx_n) * ty_n));
+        const int rem = int(tile % (long(tx_n) * ty_n));
+        const int y0 = (rem / tx_n) * TH, x0 = (rem % tx_n) * TW;
+        __syncthreads();  // the previous tile's reads are done
+        for (int c = threadIdx.x; c < HH * HW * NET_C; c += blockDim.x) {
+            const int px = c / NET_C, ci = c % NET_C;
+            const long src = halo_src(g, b, y0, x0, px / HW, px % HW);
+            xt[ci][px] = (src >= 0) ? x[src * NET_C + ci] : to_bf16(0.f);
+        }
+        for (int c = threadIdx.x; c < TH * TW * NET_C; c += blockDim.x) {
+            const int px = c / NET_C, co = c % NET_C;
+            const int yy = y0 + px / TW, xx = x0 + px % TW;
+            __bf16 v = to_bf16(0.f);
+            if (yy < g.h && xx < g.w) v = dz[((long(b) * g.h + yy) * g.w + xx) * NET_C + co];
+            dzt[co][px] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < TH * TW / 16; kk++) {
+            const int ty = kk / (TW / 16), c0 = (kk % (TW / 16)) * 16 + 8 * gk;
+            bf16x8 a, bb;
+            const int base = (ty + 1 + dy) * HW + c0 + 1 + dx;
+#pragma unroll
+            for (int e = 0; e < 8; e++) a[e] = xt[ch][base + e];
+            bb = *reinterpret_cast<const bf16x8 *>(&dzt[ch][ty * TW + c0]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc, 0, 0, 0);
+        }
+        if (t == 0 && lane < NET_C)
+            for (int px = 0; px < TH * TW; px++) bsum += from_bf16(dzt[lane][px]);
+    }
+    // D[m][n]: m = ci, n = co
+    float *dst = partial + size_t(blockIdx.x) * NET_WGRAD_PARTIAL;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int ci = 8 * (r >> 2) + 4 * gk + (r & 3), co = lane & 31;
+        dst[(t * NET_C + ci) * NET_C + co] = acc[r];
+    }
+    if (t == 0 && lane < NET_C) dst[NET_T * NET_C * NET_C + lane] = bsum;
+}
+
+// grad[co][ci][t] and grad bias from nblk partials; masked entries (frozen
+// weights) stay zero.
+__global__ void k_wgrad_reduce(const float *__restrict__ partial, int nblk, float *__restrict__ grad, int layer) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NET_LAYER_PARAMS) return;
+    int src;
+    bool live;
+    if (e < NET_C * NET_C * NET_T) {
+        const int co = e / (NET_C * NET_T), ci = (e / NET_T) % NET_C, t = e % NET_T;
+        src = (t * NET_C + ci) * NET_C + co;
+        live = !(layer == 0 && ci >= NET_CIN) && !(layer == NET_L - 1 && co >= 1);
+    } else {
+        const int co = e - NET_C * NET_C * NET_T;
+        src = NET_T * NET_C * NET_C + co;
+        live = !(layer == NET_L - 1 && co >= 1);
+    }
+    double s = 0.;
+    for (int k = 0; k < nblk; k++) s += partial[size_t(k) * NET_WGRAD_PARTIAL + src];
+    grad[size_t(layer) * NET_LAYER_PARAMS + e] = live ? float(s) : 0.f;
+}
+
+// Pack bf16 weights in the lane order of the MFMA B operand.
+//   forward:  step s = 2 t + hh, lane (n = co, gk): ci = 16 hh + 8 gk + e
+//   backward: n = ci of the layer, k = co, tap flipped (8 - t)
+__global__ void k_pack(const float *__restrict__ params, __bf16 *__restrict__ fwd, __bf16 *__restrict__ bwd) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;  // over layers x 18 x 64 x 8
+    if (idx >= NET_L * NET_PACK) return;
+    const int l = idx / NET_PACK, r = idx % NET_PACK;
+    const int s = r / 512, lane = (r / 8) % 64, e = r % 8;
+    const int t = s >> 1, hh = s & 1, n = lane & 31, gk = lane >> 5, k = 16 * hh + 8 * gk + e;
+    const float *W = params + size_t(l) * NET_LAYER_PARAMS;
+    fwd[idx] = to_bf16(W[(n * NET_C + k) * NET_T + t]);
+    bwd[idx] = to_bf16(W[(k * NET_C + n) * NET_T + (8 - t)]);
+}
+
+__global__ void k_adam(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ grad, int n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = grad[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// network input from channel-major fp32 features [b][3][h][w]
+__global__ void k_pack_input(NetGeo g, const float *__restrict__ feat, __bf16 *__restrict__ x) {
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    const long hw = long(g.h) * g.w, n = hw * g.nb;
+    if (p >= n) return;
+    const long b = p / hw, q = p % hw;
+    bf16x8 lo = {}, z = {};
+    lo[0] = to_bf16(feat[(b * 3 + 0) * hw + q]);
+    lo[1] = to_bf16(feat[(b * 3 + 1) * hw + q]);
+    lo[2] = to_bf16(feat[(b * 3 + 2) * hw + q]);
+    bf16x8 *dst = reinterpret_cast<bf16x8 *>(x + p * NET_C);
+    dst[0] = lo;
+    dst[1] = z;
+    dst[2] = z;
+    dst[3] = z;
+}
+
+// d loss / d out for loss = sum mask (out - target)^2 / msum, as the
+// channel-0 pre-activation gradient of layer 6; loss partial per block.
+__global__ void __launch_bounds__(256) k_loss(long n, const float *__restrict__ out, const float *__restrict__ target,
+                                              const float *__restrict__ mask, float inv_msum, __bf16 *__restrict__ dz,
+                                              double *__restrict__ lpart) {
+    __shared__ double red[256];
+    const long p = blockIdx.x * long(blockDim.x) + threadIdx.x;
+    double l = 0.;
+    if (p < n) {
+        const float e = out[p] - target[p];
+        l = double(mask[p]) * e * e;
+        bf16x8 z = {};
+        bf16x8 *dst = reinterpret_cast<bf16x8 *>(dz + p * NET_C);
+        bf16x8 v = z;
+        v[0] = to_bf16(2.f * mask[p] * e * inv_msum);
+        dst[0] = v;
+        dst[1] = z;
+        dst[2] = z;
+        dst[3] = z;
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+// Training crops: crop b of the batch is rows y0.. (no wrap) and columns
+// x0.. (periodic) of one recorded sample.  The loss mask leaves out a
+// border of `margin` pixels wherever the crop edge is not a domain edge
+// (the network sees zeros there instead of the neighbouring values).
+__global__ void k_gather(int ch, int cw, int margin, const CropDesc *__restrict__ d, float *__restrict__ feat,
+                         float *__restrict__ tgt, float *__restrict__ mask) {
+    const int b = blockIdx.y;
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= ch * cw) return;
+    const int py = p / cw, px = p % cw;
+    const CropDesc c = d[b];
+    const int yy = c.y0 + py, xx = (c.x0 + px) % c.N;
+    const size_t plane = size_t(c.M) * c.N, q = size_t(yy) * c.N + xx;
+    const size_t hw = size_t(ch) * cw, o = size_t(b) * hw + p;
+#pragma unroll
+    for (int k = 0; k < 3; k++) feat[(size_t(b) * 3 + k) * hw + p] = c.src[k * plane + q];
+    tgt[o] = c.src[3 * plane + q];
+    const bool in_x = px >= margin && px < cw - margin;
+    const bool in_lo = (c.y0 == 0) || py >= margin;
+    const bool in_hi = (c.y0 + ch == c.M) || py < ch - margin;
+    mask[o] = (in_x && in_lo && in_hi) ? 1.f : 0.f;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+void net_gather(int nb, int ch, int cw, int margin, const CropDesc *d, float *feat, float *tgt, float *mask,
+                hipStream_t st) {
+    hipLaunchKernelGGL(k_gather, dim3((ch * cw + 255) / 256, nb), dim3(256), 0, st, ch, cw, margin, d, feat, tgt,
+                       mask);
+}
+
+void net_pack(const float *params, __bf16 *fwd, __bf16 *bwd, hipStream_t st) {
+    const int n = NET_L * NET_PACK;
+    hipLaunchKernelGGL(k_pack, dim3((n + 255) / 256), dim3(256), 0, st, params, fwd, bwd);
+}
+
+void net_pack_input(const NetGeo &g, const float *feat, __bf16 *x, hipStream_t st) {
+    const long n = long(g.nb) * g.h * g.w;
+    hipLaunchKernelGGL(k_pack_input, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g, feat, x);
+}
+
+static dim3 conv_grid(const NetGeo &g) { return dim3((g.w + TW - 1) / TW, (g.h + TH - 1) / TH, g.nb); }
+
+void net_conv_hidden(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A, __bf16 *Z,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(k_conv<0>, conv_grid(g), dim3(256), 0, st, g, in, reinterpret_cast<const bf16x8 *>(wpk), bias,
+                       A, Z, nullptr, nullptr);
+}
+
+void net_conv_last(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, float *out,
+                   hipStream_t st) {
+    hipLaunchKernelGGL(k_conv<1>, conv_grid(g), dim3(256), 0, st, g, in, reinterpret_cast<const bf16x8 *>(wpk), bias,
+                       nullptr, nullptr, out, nullptr);
+}
+
+void net_conv_bwd(const NetGeo &g, const __bf16 *dz, const __bf16 *wpk_bwd, const __bf16 *zbelow, __bf16 *dzbelow,
+                  hipStream_t st) {
+    hipLaunchKernelGGL(k_conv<2>, conv_grid(g), dim3(256), 0, st, g, dz, reinterpret_cast<const bf16x8 *>(wpk_bwd),
+                       nullptr, dzbelow, nullptr, nullptr, zbelow);
+}
+
+void net_wgrad(const NetGeo &g, const __bf16 *x, const __bf16 *dz, float *partial, int nblk, float *grad, int layer,
+               hipStream_t st) {
+    hipLaunchKernelGGL(k_conv_wgrad, dim3(nblk), dim3(576), 0, st, g, x, dz, partial);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((NET_LAYER_PARAMS + 255) / 256), dim3(256), 0, st, partial, nblk, grad,
+                       layer);
+}
+
+void net_loss(long n, const float *out, const float *target, const float *mask, float inv_msum, __bf16 *dz,
+              double *lpart, hipStream_t st) {
+    hipLaunchKernelGGL(k_loss, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, n, out, target, mask, inv_msum, dz,
+                       lpart);
+}
+
+void net_adam(float *w, float *m, float *v, const float *grad, int n, float lr, float b1, float b2, float eps,
+              float c1, float c2, hipStream_t st) {
+    hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, w, m, v, grad, n, lr, b1, b2, eps, c1, c2);
+}
+
+}  // namespace vvmd

@@ -510,13 +510,26 @@ static void free_pcg(PCG<T> &P) {
 void Poisson::destroy() {
     free_pcg(pcg64);
     if (have32) free_pcg(pcg32);
-    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64})
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64, (void *)gfeat, (void *)gdelta,
+                    (void *)gsc, (void *)samples})
         if (p) hipFree(p);
     f = xprev = xprev2 = r64 = nullptr;
+    gfeat = gdelta = samples = nullptr;
+    gsc = nullptr;
+    sample_cap = sample_count = 0;
+    net = nullptr;
     if (hscal) hipHostFree(hscal);
     hscal = nullptr;
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
+}
+
+int Poisson::replan(size_t budget) {
+    if (budget > size_t(64) * 1024) return VVM_ERR_ARG;
+    if (mg_plan(pcg64.mg, budget) != 0) return VVM_ERR_NOMEM;
+    if (have32 && mg_plan(pcg32.mg, budget) != 0) return VVM_ERR_NOMEM;
+    lds_budget = budget;
+    return VVM_OK;
 }
 
 void Poisson::rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st) {

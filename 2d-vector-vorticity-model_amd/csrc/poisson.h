@@ -13,6 +13,7 @@
 #pragma once
 
 #include "mg.h"
+#include "net.h"
 
 namespace vvmd {
 
@@ -63,9 +64,29 @@ class Poisson {
     // FP32 MG-PCG inside FP64 iterative refinement.
     SolveResult solve_mixed(double tol, int max_outer, int max_inner, double inner_tol, hipStream_t st);
 
+    // Re-plan which multigrid levels the one-workgroup coarse kernel takes
+    // (LDS budget in bytes, at most 64 KiB); a tuning knob, the
+    // preconditioner's arithmetic is the same either way.
+    int replan(size_t lds_budget);
+    size_t lds_budget = size_t(64) * 1024;
+
     double *x64() const { return pcg64.x; }
     double *f64() const { return f; }
     int N = 0, M = 0;
+
+    // Learned initial guess (k_guess.hip).  With a network attached and two
+    // previous solutions, x = xe + s_d * net(features), xe = 2 x_{t-1} -
+    // x_{t-2}; returns false (nothing written) otherwise.
+    GuessNet *net = nullptr;
+    bool learned_guess(hipStream_t st);
+    // Sample recording for training: features before the solve, target
+    // (x - xe) / s_d after it.  cap 0 frees the buffer.
+    int collect_reserve(int cap);
+    void collect_features(hipStream_t st);  // before the solve (when recording)
+    void collect_target(hipStream_t st);    // after the solve
+    float *samples = nullptr;               // [cap][4][M][N]: 3 features, target
+    int sample_cap = 0, sample_count = 0;
+    bool sample_open = false;               // features written, target pending
 
    private:
     void vcycle64(hipStream_t st, bool first_done, double **zcur);
@@ -79,6 +100,10 @@ class Poisson {
     double *r64 = nullptr;     // FP64 residual of the refinement loop
     double *hscal = nullptr;   // pinned host copy of scalars
     RedSlot red;
+    float *gfeat = nullptr;    // [3][M][N] features of the learned guess
+    float *gdelta = nullptr;   // [M][N] network output
+    double *gsc = nullptr;     // sums of squares of the three features
+    void guess_features(float *dst, hipStream_t st);
     hipEvent_t ev = nullptr;
 };
 

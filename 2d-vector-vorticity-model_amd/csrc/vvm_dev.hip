@@ -3,7 +3,9 @@
 // timestep in the order of vvm::Iteration::TimeMarching
 // (src/Iteration.cpp:227-436).
 #include <algorithm>
+#include <array>
 #include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,6 +33,10 @@ struct vvm_dev {
     double *precip = nullptr, *xi = nullptr, *uxi = nullptr;
     double *s = nullptr;
     double *dcoef[DC_N] = {};
+    unsigned long long *dslots = nullptr;
+    int dstrip_cw = 0;              // diffusion strip width in wavefronts (0: default)
+    GuessNet *guess_net = nullptr;  // attached learned guess (not owned)
+    int guess_net_device = -1;
     Profiles P{};
     Reductions red{};
 
@@ -138,6 +144,8 @@ StepPtrs ptrs(vvm_dev *h) {
     S.s = h->s;
     S.P = h->P;
     for (int q = 0; q < DC_N; q++) S.dcoef[q] = h->dcoef[q];
+    S.dslots = h->dslots;
+    S.dstrip_cw = h->dstrip_cw;
     return S;
 }
 
@@ -222,7 +230,12 @@ int ensure_poisson(vvm_dev *h, const double *rhou, const double *rhow) {
 int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss) {
     const vvm_dev_config &c = h->cfg;
     h->pois.rhs_from_zetap(h->G, S.fut[F_ZETA], h->P.rhow, h->st);
-    h->pois.set_guess(c.guess_mode, h->st);
+    h->pois.collect_features(h->st);  // no-op unless recording samples
+    h->pois.net = h->guess_net;
+    const bool learned = (c.guess_mode == VVM_GUESS_LEARNED) && h->pois.learned_guess(h->st);
+    // without a network (or before two solutions exist) the learned mode
+    // starts from the extrapolation
+    if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP : c.guess_mode, h->st);
     SolveResult R;
     const int maxit = c.max_iter > 0 ? c.max_iter : 200;
     if (c.solver_mode == VVM_SOLVER_MIXED) {
@@ -232,6 +245,7 @@ int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss) {
         const double tol = (c.stop_mode == VVM_STOP_PARITY) ? 0. : c.tol_rel;
         R = h->pois.solve_fp64(tol, maxit, h->st);
     }
+    h->pois.collect_target(h->st);  // before x_{t-1}, x_{t-2} move on
     h->pois.remember_solution(h->st);
     h->pois.back_to_w(h->G, S.w, h->P.rhow, h->st);
     if (ss) {
@@ -313,6 +327,7 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     h->uxi = A(G.nx);
     h->s = A(NSCALAR);
     for (int q = 0; q < DC_N; q++) h->dcoef[q] = A(n2);
+    h->dslots = diffusion_slots(G, device);
     double **prof[] = {&h->P.rhou, &h->P.rhow, &h->P.pib, &h->P.pb, &h->P.thb, &h->P.thbm, &h->P.thb_init,
                        &h->P.thb_zeta, &h->P.thvb, &h->P.thvbm, &h->P.qvb, &h->P.qvb0, &h->P.qvsb, &h->P.z,
                        &h->P.z_zeta, &h->P.lambda2, &h->P.Q1LS, &h->P.Q2LS, &h->P.vtfac, &h->P.pcoef,
@@ -338,6 +353,7 @@ void vvm_dev_destroy(vvm_dev_t *h) {
     if (!h) return;
     hipSetDevice(h->device);
     if (h->st) hipStreamSynchronize(h->st);
+    diffusion_slots_release(h->dslots);
     if (h->pois_ready) h->pois.destroy();
     for (void *p : h->allocs) hipFree(p);
     if (h->pinned) hipHostFree(h->pinned);
@@ -780,4 +796,162 @@ int vvm_dev_poisson(vvm_dev_t *h, const double *b, double *x, vvm_solve_stats *s
     return rc;
 }
 
+int vvm_dev_set_guess_net(vvm_dev_t *h, vvm_net_t *net) {
+    if (!h) return VVM_ERR_ARG;
+    if (!net) {
+        h->guess_net = nullptr;
+        return VVM_OK;
+    }
+    if (guess_net_device(net) != h->device) return fail(h, VVM_ERR_ARG, "guess network lives on another device");
+    HIPCHK(h, hipSetDevice(h->device));
+    h->guess_net = guess_net_of(net);
+    return VVM_OK;
+}
+
+int vvm_dev_collect(vvm_dev_t *h, int capacity) {
+    if (!h || capacity < 0) return VVM_ERR_ARG;
+    if (!h->pois_ready) return fail(h, VVM_ERR_STATE, "collect before upload (the solver is set up on upload)");
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    const int rc = h->pois.collect_reserve(capacity);
+    return rc ? fail(h, rc, "sample buffer allocation failed") : VVM_OK;
+}
+
+int vvm_dev_collected(vvm_dev_t *h) { return (h && h->pois_ready) ? h->pois.sample_count : 0; }
+
+int vvm_dev_set_shape(vvm_dev_t *h, int cw, int kib) {
+    if (!h || cw < 0 || kib < 0 || kib > 64) return VVM_ERR_ARG;
+    if (cw) {
+        if (!diffusion_strip_waves_ok(cw)) return fail(h, VVM_ERR_ARG, "strip width must be 2..8 wavefronts");
+        if (h->dslots && (h->G.nx - 2 + cw * 64 - 1) / (cw * 64) > 64)
+            return fail(h, VVM_ERR_ARG, "too many strips for the carry slots");
+        h->dstrip_cw = cw;
+    }
+    if (kib) {
+        if (!h->pois_ready) return fail(h, VVM_ERR_STATE, "coarse budget before upload");
+        HIPCHK(h, hipSetDevice(h->device));
+        HIPCHK(h, hipStreamSynchronize(h->st));
+        const int rc = h->pois.replan(size_t(kib) * 1024);
+        if (rc) return fail(h, rc, "multigrid re-plan failed");
+    }
+    return VVM_OK;
+}
+
+// Launch-shape tuner: times whole steps for each candidate (diffusion strip
+// width x multigrid coarse-kernel LDS budget) on the live state, keeps the
+// fastest, and restores the state.  Results are cached per device, grid and
+// solver mode for the life of the process.
+int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out) {
+    if (!h || steps <= 0) return VVM_ERR_ARG;
+    if (!h->uploaded || !h->pois_ready) return fail(h, VVM_ERR_STATE, "tune before upload");
+    HIPCHK(h, hipSetDevice(h->device));
+    static std::mutex mu;
+    static std::vector<std::pair<std::array<int, 4>, vvm_tune_result>> cache;
+    const std::array<int, 4> key{h->device, h->cfg.nx, h->cfg.nz, h->cfg.solver_mode};
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        for (auto &e : cache)
+            if (e.first == key) {
+                h->dstrip_cw = e.second.diffusion_strip_waves;
+                const int rc = h->pois.replan(size_t(e.second.mg_coarse_lds_kib) * 1024);
+                if (rc) return fail(h, rc, "multigrid re-plan failed");
+                if (out) *out = e.second;
+                return VVM_OK;
+            }
+    }
+    // snapshot: the host view of the state, the solver's guess history
+    const Grid &G = h->G;
+    const size_t f2 = size_t(G.nx) * G.nz, nu = size_t(h->pois.M + 2) * h->pois.N;
+    std::vector<std::vector<double>> store;
+    auto buf = [&](size_t n) {
+        store.emplace_back(n, 0.);
+        return store.back().data();
+    };
+    vvm_host_view v{};
+    double **fields[] = {&v.zeta, &v.zetam, &v.zetap, &v.th, &v.thm, &v.thp, &v.qv, &v.qvm, &v.qvp, &v.qc, &v.qcm,
+                         &v.qcp, &v.qr, &v.qrm, &v.qrp, &v.u, &v.w, &v.RKM, &v.RKH, &v.init_th_forcing,
+                         &v.evaporation, &v.accretion, &v.autoconversion, &v.condensation};
+    store.reserve(64);
+    for (double **p : fields) *p = buf(f2);
+    double **hists[] = {&v.dth_advect, &v.dth_buoyancy, &v.dzeta_advect, &v.dqv_advect, &v.dqc_advect, &v.dqr_advect,
+                        &v.dqr_VT};
+    for (double **p : hists) *p = buf(2 * f2);
+    double **rows[] = {&v.precip, &v.xi, &v.uxi};
+    for (double **p : rows) *p = buf(G.nx);
+    double **profs[] = {&v.rhou, &v.rhow, &v.pib, &v.pb, &v.thb, &v.thbm, &v.thb_init, &v.thb_zeta, &v.thvb,
+                        &v.thvbm, &v.qvb, &v.qvb0, &v.qvsb, &v.z, &v.z_zeta, &v.lambda2, &v.Q1LS, &v.Q2LS};
+    for (double **p : profs) *p = buf(G.nz);
+    int rc = vvm_dev_download(h, &v);
+    if (rc) return rc;
+    double *gs = nullptr;
+    HIPCHK(h, hipMalloc(&gs, 3 * nu * sizeof(double)));
+    hipMemcpyAsync(gs, h->pois.x64(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    hipMemcpyAsync(gs + nu, h->pois.xprev_ptr(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    hipMemcpyAsync(gs + 2 * nu, h->pois.xprev2_ptr(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    const int nsolves = h->pois.nsolves, scap = h->pois.sample_cap;
+    h->pois.sample_cap = h->pois.sample_count;  // record nothing while tuning
+
+    const int cws[] = {2, 4, 8};
+    const int kib[] = {16, 32, 64};
+    vvm_tune_result best{};
+    best.ms_per_step = 1e300;
+    double ms_default = 0.;
+    for (int cw : cws) {
+        if (!h->dslots && cw != 4) continue;  // one-workgroup sweep: nothing to choose
+        if (h->dslots && (G.nx - 2 + cw * 64 - 1) / (cw * 64) > 64) continue;
+        for (int kb : kib) {
+            h->dstrip_cw = cw;
+            if ((rc = h->pois.replan(size_t(kb) * 1024)) != VVM_OK) break;
+            if ((rc = vvm_dev_step(h, 1, nullptr)) < 0) break;  // warm-up
+            hipEventRecord(h->ev[0], h->st);
+            if ((rc = vvm_dev_step(h, steps, nullptr)) < 0) break;
+            hipEventRecord(h->ev[1], h->st);
+            hipEventSynchronize(h->ev[1]);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+            const double per = double(ms) / steps;
+            if (cw == 4 && kb == 64) ms_default = per;
+            if (per < best.ms_per_step) {
+                best.ms_per_step = per;
+                best.diffusion_strip_waves = cw;
+                best.mg_coarse_lds_kib = kb;
+            }
+            rc = vvm_dev_upload(h, &v);  // back to the snapshot for the next candidate
+            if (rc) break;
+            hipMemcpyAsync(h->pois.xprev_ptr(), gs + nu, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+            hipMemcpyAsync(h->pois.xprev2_ptr(), gs + 2 * nu, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+            h->pois.nsolves = nsolves;
+        }
+        if (rc < 0) break;
+    }
+    hipMemcpyAsync(h->pois.x64(), gs, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    hipStreamSynchronize(h->st);
+    hipFree(gs);
+    h->pois.sample_cap = scap;
+    if (rc < 0) return rc;
+    best.ms_default = ms_default;
+    h->dstrip_cw = best.diffusion_strip_waves;
+    rc = h->pois.replan(size_t(best.mg_coarse_lds_kib) * 1024);
+    if (rc) return fail(h, rc, "multigrid re-plan failed");
+    {
+        std::lock_guard<std::mutex> lk(mu);
+        cache.push_back({key, best});
+    }
+    if (out) *out = best;
+    return VVM_OK;
+}
+
 }  // extern "C"
+
+namespace vvmd {
+int dev_samples(vvm_dev_t *h, const float **buf, int *N, int *M, int *count, int *device) {
+    if (!h || !h->pois_ready) return VVM_ERR_STATE;
+    if (hipSetDevice(h->device) != hipSuccess || hipStreamSynchronize(h->st) != hipSuccess) return VVM_ERR_HIP;
+    *buf = h->pois.samples;
+    *N = h->pois.N;
+    *M = h->pois.M;
+    *count = h->pois.sample_count;
+    *device = h->device;
+    return VVM_OK;
+}
+}  // namespace vvmd

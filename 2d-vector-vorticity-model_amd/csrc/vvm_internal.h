@@ -53,6 +53,10 @@ struct StepPtrs {
     // parallel before the sequential sweep: a, d of Mparam (zeta) and of
     // Hparam (shared by th, qv, qc, qr), and c of each field.
     double *dcoef[DC_N];
+    // carry slots of the multi-workgroup diffusion sweep (uncached device
+    // memory, [field][strip][k]); nullptr: the one-workgroup sweep runs
+    unsigned long long *dslots;
+    int dstrip_cw;  // compute wavefronts per strip of that sweep (0: default)
 };
 
 struct Reductions {
@@ -77,6 +81,11 @@ void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st);
 bool diffusion_supported(const Grid &G);  // nx - 2 <= 8192
 // nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta into the sweep's store.
 void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st);
+// Carry slots for the multi-workgroup sweep on `device` (allocated once per
+// process and device, never freed); nullptr when unavailable.
+unsigned long long *diffusion_slots(const Grid &G, int device);
+void diffusion_slots_release(unsigned long long *p);
+bool diffusion_strip_waves_ok(int cw);
 void launch_nudge(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st);
 void launch_nudge_qv(const Grid &G, const StepPtrs &S, double tau, hipStream_t st);
 void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipStream_t st);

@@ -54,6 +54,7 @@ PROFILES = ["rhou", "rhow", "pib", "pb", "thb", "thbm", "thb_init", "thb_zeta", 
             "qvb", "qvb0", "qvsb", "z", "z_zeta", "lambda2", "Q1LS", "Q2LS"]
 SCALARS = ["ubarTop", "ubarTopp", "ubarTopm"]
 PTR = C.POINTER(C.c_double)
+FPTR = C.POINTER(C.c_float)
 
 
 class VvmHostView(C.Structure):
@@ -80,6 +81,22 @@ class VvmCaseParams(C.Structure):
                 ("bubble_radius", C.c_double), ("bubble_zc", C.c_double), ("noise_kind", C.c_int32),
                 ("noise_amp", C.c_double), ("noise_ztop", C.c_double), ("seed", C.c_uint64),
                 ("sfc_heat_flux", C.c_double), ("sfc_qv_ramp", C.c_double), ("sfc_qv_ramp_z", C.c_double)]
+
+
+class VvmTuneResult(C.Structure):
+    _fields_ = [("diffusion_strip_waves", C.c_int32), ("mg_coarse_lds_kib", C.c_int32), ("ms_per_step", C.c_double),
+                ("ms_default", C.c_double)]
+
+
+class VvmTrainOpts(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("batch", C.c_int32), ("crop_h", C.c_int32), ("crop_w", C.c_int32),
+                ("margin", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float), ("seed", C.c_uint64),
+                ("log_every", C.c_int32)]
+
+
+class VvmTrainStats(C.Structure):
+    _fields_ = [("loss_first", C.c_double), ("loss_last", C.c_double), ("seconds", C.c_double),
+                ("steps", C.c_int32)]
 
 
 _LIB = None
@@ -111,6 +128,30 @@ def load_lib(path: str = LIB_PATH):
                                               C.c_double, C.c_double, C.c_double]),
         "vvm_field_len": (C.c_int64, [C.c_int, C.c_int]),
         "vvm_hist_len": (C.c_int64, [C.c_int, C.c_int]),
+        # learned initial guess (include/vvm_net.h)
+        "vvm_net_create": (C.c_int, [C.c_int, P(C.c_void_p)]),
+        "vvm_net_destroy": (None, [C.c_void_p]),
+        "vvm_net_last_error": (C.c_char_p, [C.c_void_p]),
+        "vvm_net_param_count": (C.c_int, []),
+        "vvm_net_set_params": (C.c_int, [C.c_void_p, FPTR]),
+        "vvm_net_get_params": (C.c_int, [C.c_void_p, FPTR]),
+        "vvm_net_get_grad": (C.c_int, [C.c_void_p, FPTR]),
+        "vvm_net_init_params": (C.c_int, [C.c_void_p, C.c_uint64]),
+        "vvm_net_forward_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, FPTR, FPTR]),
+        "vvm_net_grad_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, FPTR, FPTR, FPTR, FPTR,
+                                        P(C.c_double)]),
+        "vvm_net_adam": (C.c_int, [C.c_void_p, C.c_float]),
+        "vvm_dev_collect": (C.c_int, [C.c_void_p, C.c_int]),
+        "vvm_dev_collected": (C.c_int, [C.c_void_p]),
+        "vvm_net_add_samples": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "vvm_net_pool_size": (C.c_int, [C.c_void_p]),
+        "vvm_net_train": (C.c_int, [C.c_void_p, P(VvmTrainOpts), P(VvmTrainStats)]),
+        "vvm_net_eval": (C.c_int, [C.c_void_p, C.c_void_p, P(C.c_double)]),
+        "vvm_net_save": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "vvm_net_load": (C.c_int, [C.c_void_p, C.c_char_p]),
+        "vvm_dev_set_guess_net": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "vvm_dev_tune": (C.c_int, [C.c_void_p, C.c_int, P(VvmTuneResult)]),
+        "vvm_dev_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name, None)
@@ -351,6 +392,27 @@ class Device:
     def op(self, name: str) -> int:
         return self._check(self.lib.vvm_dev_op(self.h, OPS[name]), f"op {name}")
 
+    def tune(self, steps: int = 3) -> "VvmTuneResult":
+        r = VvmTuneResult()
+        self._check(self.lib.vvm_dev_tune(self.h, steps, C.byref(r)), "tune")
+        return r
+
+    def set_strip_waves(self, cw: int) -> None:
+        self._check(self.lib.vvm_dev_set_shape(self.h, cw, 0), "set_shape")
+
+    def set_shape(self, cw: int = 0, coarse_kib: int = 0) -> None:
+        self._check(self.lib.vvm_dev_set_shape(self.h, cw, coarse_kib), "set_shape")
+
+    def collect(self, capacity: int) -> None:
+        """Record training samples of the learned guess (after upload)."""
+        self._check(self.lib.vvm_dev_collect(self.h, capacity), "collect")
+
+    def collected(self) -> int:
+        return int(self.lib.vvm_dev_collected(self.h))
+
+    def set_guess_net(self, net: "Net | None") -> None:
+        self._check(self.lib.vvm_dev_set_guess_net(self.h, net.h if net is not None else None), "set_guess_net")
+
     def poisson(self, b: np.ndarray, x: np.ndarray) -> VvmSolveStats:
         s = VvmSolveStats()
         assert b.dtype == np.float64 and x.dtype == np.float64 and b.flags.c_contiguous and x.flags.c_contiguous
@@ -367,6 +429,102 @@ class Device:
     def close(self) -> None:
         if self.h:
             self.lib.vvm_dev_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Net:
+    """The learned initial-guess network on one device (include/vvm_net.h)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_lib()
+        self.h = C.c_void_p()
+        rc = self.lib.vvm_net_create(device, C.byref(self.h))
+        if rc != 0:
+            raise VvmError(f"vvm_net_create failed ({rc})")
+        self.n = int(self.lib.vvm_net_param_count())
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise VvmError(f"{what} failed ({rc}): {self.lib.vvm_net_last_error(self.h).decode()}")
+        return rc
+
+    def init_params(self, seed: int) -> None:
+        self._check(self.lib.vvm_net_init_params(self.h, C.c_uint64(seed)), "init_params")
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.n
+        self._check(self.lib.vvm_net_set_params(self.h, p.ctypes.data_as(FPTR)), "set_params")
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n, dtype=np.float32)
+        self._check(self.lib.vvm_net_get_params(self.h, p.ctypes.data_as(FPTR)), "get_params")
+        return p
+
+    def forward(self, inp: np.ndarray, periodic: bool) -> np.ndarray:
+        x = np.ascontiguousarray(inp, dtype=np.float32)
+        nb, c, h, w = x.shape
+        assert c == 3
+        out = np.zeros((nb, h, w), dtype=np.float32)
+        self._check(self.lib.vvm_net_forward_host(self.h, nb, h, w, int(periodic), x.ctypes.data_as(FPTR),
+                                                  out.ctypes.data_as(FPTR)), "forward")
+        return out
+
+    def grad(self, inp, target, mask, periodic: bool):
+        x = np.ascontiguousarray(inp, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        nb, _, h, w = x.shape
+        g = np.zeros(self.n, dtype=np.float32)
+        loss = C.c_double(0.)
+        self._check(self.lib.vvm_net_grad_host(self.h, nb, h, w, int(periodic), x.ctypes.data_as(FPTR),
+                                               t.ctypes.data_as(FPTR), m.ctypes.data_as(FPTR),
+                                               g.ctypes.data_as(FPTR), C.byref(loss)), "grad")
+        return loss.value, g
+
+    def adam(self, lr: float) -> None:
+        self._check(self.lib.vvm_net_adam(self.h, C.c_float(lr)), "adam")
+
+    def add_samples(self, dev: "Device") -> None:
+        self._check(self.lib.vvm_net_add_samples(self.h, dev.h), "add_samples")
+
+    def pool_size(self) -> int:
+        return int(self.lib.vvm_net_pool_size(self.h))
+
+    def train(self, steps: int, batch: int = 16, crop: tuple = (64, 64), margin: int = 6, lr: float = 2e-3,
+              lr_final: float = 1e-4, seed: int = 1, log_every: int = 100) -> VvmTrainStats:
+        o = VvmTrainOpts(steps=steps, batch=batch, crop_h=crop[0], crop_w=crop[1], margin=margin, lr=lr,
+                         lr_final=lr_final, seed=seed, log_every=log_every)
+        s = VvmTrainStats()
+        self._check(self.lib.vvm_net_train(self.h, C.byref(o), C.byref(s)), "train")
+        return s
+
+    def eval(self, dev: "Device") -> float:
+        e = C.c_double(0.)
+        self._check(self.lib.vvm_net_eval(self.h, dev.h, C.byref(e)), "eval")
+        return e.value
+
+    def save(self, path: str) -> None:
+        self._check(self.lib.vvm_net_save(self.h, path.encode()), "save")
+
+    def load(self, path: str) -> None:
+        self._check(self.lib.vvm_net_load(self.h, path.encode()), "load")
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.vvm_net_destroy(self.h)
             self.h = C.c_void_p()
 
     def __enter__(self):

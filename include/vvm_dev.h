@@ -185,6 +185,24 @@ int vvm_dev_op(vvm_dev_t *h, int op);
 int vvm_dev_kernel_timer(vvm_dev_t *h, int kernel);
 int vvm_dev_kernel_time(vvm_dev_t *h, double *avg_seconds, int *launches);
 
+/* Launch-shape autotuner (no counterpart in the reference): times `steps`
+ * whole steps for each candidate launch shape on the live state and keeps
+ * the fastest; the model state is restored afterwards.  Candidates: the
+ * diffusion sweep's strip width (2, 4 or 8 wavefronts of 64 columns) and
+ * the LDS budget (16, 32, 64 KiB) that decides which multigrid levels run in
+ * the one-workgroup coarse kernel.  Results are cached per device, grid and
+ * solver mode for the life of the process. */
+typedef struct vvm_tune_result {
+    int32_t diffusion_strip_waves;
+    int32_t mg_coarse_lds_kib;
+    double ms_per_step;      /* the chosen shape                             */
+    double ms_default;       /* the default shape (4 waves, 64 KiB)          */
+} vvm_tune_result;
+int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out);
+/* Set the same launch shapes by hand (0 keeps the current value); takes
+ * effect from the next step.  The coarse budget needs an uploaded state. */
+int vvm_dev_set_shape(vvm_dev_t *h, int diffusion_strip_waves, int mg_coarse_lds_kib);
+
 /* Host-side helpers (no GPU use). */
 
 /* Fill the analytic base state and initial fields of one of the benchmark

@@ -90,3 +90,48 @@ def run_ref(mode: str, arg: str, in_path: str, out_path: str, timeout: float = 6
     """Run the compiled reference harness (oracle/_ref/vvm_ref), single thread."""
     env = dict(os.environ, OMP_NUM_THREADS="1")
     subprocess.run([REF_BIN, mode, str(arg), in_path, out_path], check=True, env=env, timeout=timeout)
+
+
+# ------------------------------------------------------- guess network
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def net_forward(params, inp, periodic: bool):
+    """CPU forward of the guess network: inp [nb][3][h][w] -> [nb][h][w]."""
+    import numpy as np
+    lib = load()
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(inp, dtype=np.float32)
+    nb, _, h, w = x.shape
+    out = np.zeros((nb, h, w), dtype=np.float32)
+    rc = lib.vo_net_forward(nb, h, w, int(periodic), _fp(p), _fp(x), _fp(out))
+    if rc != 0:
+        raise RuntimeError(f"vo_net_forward returned {rc}")
+    return out
+
+
+def net_grad(params, inp, target, mask, periodic: bool, rounding: bool = True):
+    """CPU loss and parameter gradient of the guess network."""
+    import numpy as np
+    lib = load()
+    lib.vo_net_set_rounding(int(rounding))
+    try:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        x = np.ascontiguousarray(inp, dtype=np.float32)
+        t = np.ascontiguousarray(target, dtype=np.float32)
+        m = np.ascontiguousarray(mask, dtype=np.float32)
+        nb, _, h, w = x.shape
+        g = np.zeros_like(p)
+        loss = C.c_double(0.)
+        rc = lib.vo_net_grad(nb, h, w, int(periodic), _fp(p), _fp(x), _fp(t), _fp(m), _fp(g), C.byref(loss))
+        if rc != 0:
+            raise RuntimeError(f"vo_net_grad returned {rc}")
+        return loss.value, g
+    finally:
+        lib.vo_net_set_rounding(1)
+
+
+def net_param_count() -> int:
+    return int(load().vo_net_param_count())

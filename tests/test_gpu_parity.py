@@ -159,3 +159,45 @@ def test_one_step_parity_wide(oracle):
         got = st.copy()
         dev.download(got)
     assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="2048x64 2 steps")
+
+
+@pytest.mark.parametrize("cw", [2, 4, 8])
+def test_diffusion_strip_widths(oracle, cw):
+    """The multi-workgroup sweep at every strip width the tuner can pick,
+    including a partial last strip (1000 columns)."""
+    c = V.grid_case(1000, 40, dx=100.0, dt=1.0)
+    st = perturbed_state(c, seed=cw, step=5)
+    for f in ("zeta", "th", "qv", "qc", "qr"):
+        st[f + "p"][:] = st[f]
+    st["qcp"][:] = np.abs(st["qcp"])
+    want = st.copy()
+    oracle.op("turbulence", c.cfg, want)
+    with V.Device(c.cfg) as dev:
+        dev.set_strip_waves(cw)
+        got = gpu_run_op(dev, st, "turbulence")
+    assert_fields_close(got, want, ["RKM", "RKH", "zetap", "thp", "qvp", "qcp", "qrp"], 1e-12,
+                        where=f"turbulence strips of {cw} waves")
+
+
+def test_tuner_keeps_state_and_parity(oracle):
+    """vvm_dev_tune runs candidate shapes on the live state, restores it, and
+    the tuned handle still matches the oracle step for step."""
+    c = V.grid_case(1024, 64, dx=100.0, dt=1.0, stop_mode=V.VVM_STOP_PARITY)
+    st = V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 2)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        r = dev.tune(2)
+        print(f"tuned 1024x64: strip {r.diffusion_strip_waves} waves, coarse {r.mg_coarse_lds_kib} KiB, "
+              f"{r.ms_per_step:.3f} ms/step (default {r.ms_default:.3f})", flush=True)
+        assert r.diffusion_strip_waves in (2, 4, 8) and r.mg_coarse_lds_kib in (16, 32, 64)
+        back = st.copy()
+        dev.download(back)
+        for n in ("zeta", "th", "qv", "u", "w"):
+            np.testing.assert_array_equal(back[n], st[n], err_msg=n)
+        assert back.step == st.step
+        dev.step(2)
+        got = st.copy()
+        dev.download(got)
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="tuned 1024x64 2 steps")

@@ -1,0 +1,223 @@
+"""The learned initial-guess network: the CPU oracle (oracle/convnet.c)
+pinned by finite differences and closed forms, then the device's MFMA
+forward and backward kernels against it."""
+import numpy as np
+import pytest
+
+import vvm_amd as V
+
+NC, NT, NL = 32, 9, 6
+LP = NC * NC * NT + NC
+
+
+def layer(p, l):
+    W = p[l * LP:l * LP + NC * NC * NT].reshape(NC, NC, NT)
+    b = p[l * LP + NC * NC * NT:(l + 1) * LP]
+    return W, b
+
+
+def random_params(seed, scale=1.0):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NL * LP, dtype=np.float32)
+    for l in range(NL):
+        W, b = layer(p, l)
+        cin = 3 if l == 0 else NC
+        cout = 1 if l == NL - 1 else NC
+        W[:cout, :cin, :] = rng.normal(0, scale * np.sqrt(2.0 / (cin * NT)), (cout, cin, NT))
+        b[:cout] = rng.normal(0, 0.1, cout)
+    return p
+
+
+def sample(seed, nb, h, w):
+    rng = np.random.default_rng(seed)
+    inp = rng.normal(0, 1, (nb, 3, h, w)).astype(np.float32)
+    tgt = rng.normal(0, 1, (nb, h, w)).astype(np.float32)
+    mask = np.zeros((nb, h, w), dtype=np.float32)
+    mask[:, 1:-1, 2:-2] = 1.0
+    return inp, tgt, mask
+
+
+def gelu(z):
+    return 0.5 * z * (1 + np.tanh(0.7978845608028654 * (z + 0.044715 * z ** 3)))
+
+
+# ----------------------------------------------------------------- oracle
+
+def test_oracle_identity_network(oracle):
+    """Centre-tap weights that pass channel 0 through every layer: the output
+    is the composition of GELUs of the input's channel 0 (closed form)."""
+    p = np.zeros(NL * LP, dtype=np.float32)
+    for l in range(NL):
+        W, _ = layer(p, l)
+        W[0, 0, 4] = 1.0
+    inp, _, _ = sample(1, 2, 5, 7)
+    inp = np.abs(inp)  # keep GELU near identity and away from bf16 rounding ties
+    out = oracle.net_forward(p, inp, periodic=True)
+    lib = oracle.load()
+    import ctypes as C
+    lib.vo_bf16_round.restype = C.c_float
+    lib.vo_bf16_round.argtypes = [C.c_float]
+    r = np.vectorize(lambda x: lib.vo_bf16_round(float(x)))
+    want = r(inp[:, 0]).astype(np.float64)
+    for _ in range(NL - 1):
+        want = r(gelu(want))
+    np.testing.assert_allclose(out, want, rtol=1e-6, atol=1e-7)
+
+
+def test_oracle_shift_taps(oracle):
+    """One tap per layer moves channel 0 by one column; five such layers and
+    an identity last layer shift the input by five columns (periodic wrap),
+    or pad with zeros when the crop is not periodic."""
+    p = np.zeros(NL * LP, dtype=np.float32)
+    for l in range(NL):
+        W, _ = layer(p, l)
+        W[0, 0, 5 if l < NL - 1 else 4] = 1.0  # tap (dz=0, dx=+1): out[x] = in[x+1]
+    inp, _, _ = sample(2, 1, 4, 9)
+    inp = np.abs(inp) + 3.0  # GELU(z) = z to bf16 precision for z >= 3 is not exact; compare shapes only
+    outp = oracle.net_forward(p, inp, periodic=True)
+    outz = oracle.net_forward(p, inp, periodic=False)
+    # the non-periodic result is the periodic one with the last 5 columns cut
+    np.testing.assert_array_equal(outz[..., :4], outp[..., :4])
+    assert np.all(outz[..., 4:] == 0.0)
+    assert np.all(outp[..., 4:] != 0.0)
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_oracle_gradient_finite_differences(oracle, periodic):
+    """Analytic parameter gradient of the oracle (storage rounding off)
+    against central differences in a sample of coordinates of every layer."""
+    p = random_params(3, scale=0.5)
+    inp, tgt, mask = sample(4, 1, 5, 6)
+    loss, g = oracle.net_grad(p, inp, tgt, mask, periodic, rounding=False)
+    rng = np.random.default_rng(5)
+    checked = 0
+    for l in range(NL):
+        cin = 3 if l == 0 else NC
+        cout = 1 if l == NL - 1 else NC
+        for _ in range(4):
+            co, ci, t = rng.integers(cout), rng.integers(cin), rng.integers(NT)
+            for idx in (l * LP + (co * NC + ci) * NT + t, l * LP + NC * NC * NT + co):
+                h = 2e-3 * max(1.0, abs(float(p[idx])))
+                pp, pm = p.copy(), p.copy()
+                pp[idx] += h
+                pm[idx] -= h
+                lp, _ = oracle.net_grad(pp, inp, tgt, mask, periodic, rounding=False)
+                lm, _ = oracle.net_grad(pm, inp, tgt, mask, periodic, rounding=False)
+                fd = (lp - lm) / (float(pp[idx]) - float(pm[idx]))
+                assert abs(fd - g[idx]) <= 2e-3 * max(1.0, abs(fd)) + 1e-5, (l, idx, fd, g[idx])
+                checked += 1
+    assert checked == NL * 8
+
+
+def test_oracle_frozen_weights_have_no_gradient(oracle):
+    p = random_params(6)
+    inp, tgt, mask = sample(7, 1, 4, 5)
+    _, g = oracle.net_grad(p, inp, tgt, mask, True)
+    W0, _ = layer(g, 0)
+    W5, b5 = layer(g, NL - 1)
+    assert np.all(W0[:, 3:, :] == 0) and np.all(W5[1:] == 0) and np.all(b5[1:] == 0)
+
+
+# ----------------------------------------------------------------- device
+
+# bf16 operands with fp32 sums in another order: a product that lands near a
+# bf16 rounding boundary can round the other way, so whole-tensor relative
+# L2 differences of a few 1e-3 are expected.
+FWD_TOL = 1e-2
+GRAD_TOL = 3e-2
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(np.asarray(b, np.float64)), 1e-30))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,periodic", [((2, 6, 70), True), ((3, 64, 64), False), ((1, 127, 200), True)])
+def test_device_forward_matches_oracle(oracle, shape, periodic):
+    nb, h, w = shape
+    p = random_params(10 + h)
+    inp, _, _ = sample(11 + w, nb, h, w)
+    want = oracle.net_forward(p, inp, periodic)
+    print(f"net forward {shape} periodic={periodic}", flush=True)
+    with V.Net() as net:
+        net.set_params(p)
+        got = net.forward(inp, periodic)
+    assert rel(got, want) <= FWD_TOL, rel(got, want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("shape,periodic", [((2, 8, 70), False), ((1, 20, 64), True)])
+def test_device_gradient_matches_oracle(oracle, shape, periodic):
+    """The device's backward kernels (data and weight gradients of every
+    layer) against the CPU oracle's gradient, per layer."""
+    nb, h, w = shape
+    p = random_params(20 + h)
+    inp, tgt, mask = sample(21 + w, nb, h, w)
+    lw, gw = oracle.net_grad(p, inp, tgt, mask, periodic)
+    print(f"net gradient {shape} periodic={periodic}", flush=True)
+    with V.Net() as net:
+        net.set_params(p)
+        lg, gg = net.grad(inp, tgt, mask, periodic)
+    assert abs(lg - lw) <= 1e-2 * abs(lw)
+    for l in range(NL):
+        a, b = gg[l * LP:(l + 1) * LP], gw[l * LP:(l + 1) * LP]
+        assert rel(a, b) <= GRAD_TOL, (l, rel(a, b))
+
+
+@pytest.mark.gpu
+def test_device_adam_reduces_loss():
+    """A few hundred Adam steps on one fixed batch drive the loss down."""
+    inp, tgt, mask = sample(30, 4, 32, 64)
+    tgt = 0.5 * inp[:, 0] - 0.25 * np.roll(inp[:, 1], 1, axis=2)  # learnable local map
+    with V.Net() as net:
+        net.init_params(1)
+        l0, _ = net.grad(inp, tgt, mask, False)
+        for _ in range(300):
+            net.grad(inp, tgt, mask, False)
+            net.adam(2e-3)
+        l1, _ = net.grad(inp, tgt, mask, False)
+    assert l1 < 0.05 * l0, (l0, l1)
+
+
+# ------------------------------------------------------------ learned guess
+
+def _run(seed, steps, guess, net=None, collect=0):
+    c = V.grid_case(512, 64, seed=seed, guess_mode=guess, stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400)
+    st = V.init_state(c)
+    dev = V.Device(c.cfg)
+    dev.upload(st)
+    if collect:
+        dev.collect(collect)
+    if net is not None:
+        dev.set_guess_net(net)
+    s = dev.step(steps)
+    return dev, s
+
+
+@pytest.mark.gpu
+def test_learned_guess_gain():
+    """Train the network on samples the model records from its own runs
+    (512x64, seeds 1 and 2), then on a held-out seed: the network's guess
+    must leave less error than the extrapolation it corrects, and the solve
+    from it must still reach ||b - Ax|| / ||b|| <= 1e-10."""
+    with V.Net() as net:
+        net.init_params(1)
+        for seed in (1, 2):
+            dev, _ = _run(seed, 80, V.VVM_GUESS_EXTRAP, collect=80)
+            assert dev.collected() == 78  # from the third solve on
+            net.add_samples(dev)
+            dev.close()
+        held, _ = _run(7, 50, V.VVM_GUESS_EXTRAP, collect=50)
+        e0 = net.eval(held)
+        print(f"training on {net.pool_size()} samples", flush=True)
+        ts = net.train(steps=1500, batch=16, crop=(48, 64), lr=2e-3, lr_final=1e-4, seed=3, log_every=100)
+        e1 = net.eval(held)
+        held.close()
+        _, s_ex = _run(7, 50, V.VVM_GUESS_EXTRAP)
+        _, s_le = _run(7, 50, V.VVM_GUESS_LEARNED, net=net)
+        print(f"loss {ts.loss_first:.4g} -> {ts.loss_last:.4g} in {ts.seconds:.2f} s; held-out error "
+              f"relative to extrapolation {e0:.4f} -> {e1:.4f}; iterations over 50 steps: extrapolation "
+              f"{s_ex.solve_iters_total}, learned {s_le.solve_iters_total}", flush=True)
+    assert ts.loss_last < ts.loss_first
+    assert e1 < 0.9 * e0 and e1 < 0.9
+    assert s_le.not_converged == 0 and s_le.rel_residual_last <= 1e-10
