@@ -116,15 +116,17 @@ template <int E>
 __device__ inline void load_row(const Grid &G, const double *__restrict__ A_, const double *__restrict__ C_,
                                 const double *__restrict__ D_, int i0, int n, int k, int nz, double (&a)[E],
                                 double (&c)[E], double (&d)[E]) {
+    // Unconditional loads from a clamped (valid) position: a load under a
+    // branch merges with a default value, and that merge makes the compiler
+    // wait for the load at once, which would serialise the prefetch.  Values
+    // of positions past the last row or column are never used.
+    const int kk = min(k, nz - 2);
 #pragma unroll
     for (int j = 0; j < E; j++) {
-        a[j] = c[j] = d[j] = 0.;
-        if (k <= nz - 2 && i0 + j <= n) {
-            const size_t p = ix(G, i0 + j, k);
-            a[j] = A_[p];
-            c[j] = C_[p];
-            d[j] = D_[p];
-        }
+        const size_t p = ix(G, min(i0 + j, n), kk);
+        a[j] = A_[p];
+        c[j] = C_[p];
+        d[j] = D_[p];
     }
 }
 
@@ -300,16 +302,31 @@ __device__ void courier(const unsigned long long *in, unsigned long long *out, u
     }
 }
 
-template <int E>
-__global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid G, StepPtrs S, int nudge,
-                                                                          int nstrip) {
-    extern __shared__ unsigned long long lds[];  // carry[cw][nz], cin[nz]
-    const int cw = blockDim.x / kWave - 1;
+template <int E, int CW>
+__global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPtrs S, int nudge, int nstrip) {
+    // LDS: carry[cw][nz], cin[nz] (u64), then ghost[nz], rate[nz], thbi[nz]
+    // (doubles): the per-row values a compute wave reads come from LDS, so
+    // its only global loads are the prefetched coefficient rows (a global
+    // load consumed at once would make the wave wait for every earlier,
+    // prefetched load as well)
+    extern __shared__ unsigned long long lds[];
+    constexpr int cw = CW;
     const int nz = G.nz, n = G.nx - 2;
     unsigned long long *carry = lds, *cin = lds + size_t(cw) * nz;
-    for (int t = threadIdx.x; t < (cw + 1) * nz; t += blockDim.x) lds[t] = kEmpty;
-    __syncthreads();
+    double *ghost = reinterpret_cast<double *>(lds + size_t(cw + 1) * nz), *rate_l = ghost + nz, *thbi = rate_l + nz;
     const int s = blockIdx.x, fld = blockIdx.y;
+    {
+        const bool mz = (fld == F_ZETA);
+        const double *rt = mz ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
+        const double *fv = S.fut[fld];
+        for (int t = threadIdx.x; t < (cw + 1) * nz; t += blockDim.x) lds[t] = kEmpty;
+        for (int k = threadIdx.x; k < nz; k += blockDim.x) {
+            ghost[k] = fv[ix(G, 0, k)];
+            rate_l[k] = rt ? rt[k] : 0.;
+            thbi[k] = S.P.thb_init[k];
+        }
+    }
+    __syncthreads();
     const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
     const unsigned long long *in = (s > 0) ? S.dslots + (size_t(fld) * nstrip + (s - 1)) * nz : nullptr;
     unsigned long long *out = (s + 1 < nstrip) ? S.dslots + (size_t(fld) * nstrip + s) * nz : nullptr;
@@ -322,7 +339,6 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
     const double *__restrict__ C_ = S.dcoef[DC_C0 + fld];
     const double *__restrict__ D_ = S.dcoef[mpar ? DC_DM : DC_DH];
     double *f = S.fut[fld];
-    const double *rate = mpar ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
     const bool is_theta = (fld == F_TH);
     const int W = cw * kWave * E;
     const int strip0 = 1 + s * W + w * kWave * E;       // first column of this wave
@@ -337,15 +353,19 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
     unsigned long long *dst = carry + size_t(w) * nz;
     const bool publish = (w < cw - 1) || (out != nullptr);
 
-    double ba[PF][E], bc[PF][E], bd[PF][E], ynew[E];
+    // rows in flight per lane: a wave's row takes well under a memory
+    // latency, so the loads run PFS rows ahead (8; 4 for the widest strip,
+    // whose nine wavefronts leave fewer registers per lane)
+    constexpr int PFS = (CW >= 8) ? 4 : 8;
+    double ba[PFS][E], bc[PFS][E], bd[PFS][E], ynew[E];
 #pragma unroll
     for (int j = 0; j < E; j++) ynew[j] = 0.;
 #pragma unroll
-    for (int q = 0; q < PF; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
+    for (int q = 0; q < PFS; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
     bool gave_up = false;
-    for (int k0 = 1; k0 <= nz - 2; k0 += PF) {
+    for (int k0 = 1; k0 <= nz - 2; k0 += PFS) {
 #pragma unroll
-        for (int q = 0; q < PF; q++) {
+        for (int q = 0; q < PFS; q++) {
             const int k = k0 + q;
             if (k > nz - 2) break;
             double a[E], e[E];
@@ -356,7 +376,7 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
                 e[j] = bc[q][j] + bd[q][j] * ynew[j];
                 if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
             }
-            load_row<E>(G, A_, C_, D_, i0, n, k + PF, nz, ba[q], bc[q], bd[q]);
+            load_row<E>(G, A_, C_, D_, i0, n, k + PFS, nz, ba[q], bc[q], bd[q]);
             Affine inc = m;
 #pragma unroll
             for (int off = 1; off < kWave; off <<= 1) {
@@ -366,7 +386,7 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
             const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
             double y0;
             if (!src) {
-                y0 = f[ix(G, 0, k)];  // ghost column, old as in the reference
+                y0 = ghost[k];  // ghost column, old as in the reference
             } else {
                 unsigned long long bits = kEmpty;
                 if (lane == 0) {
@@ -382,7 +402,7 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
                 y0 = __shfl(__longlong_as_double(static_cast<long long>(bits)), 0, kWave);
             }
             double y = (lane == 0) ? y0 : ea * y0 + eb;
-            const double r = rate ? rate[k] : 0.;
+            const double r = rate_l[k];
 #pragma unroll
             for (int j = 0; j < E; j++) {
                 const int i = i0 + j;
@@ -390,7 +410,7 @@ __global__ void __launch_bounds__((kMaxCompute + 1) * kWave) k_diff_strips(Grid 
                 y = a[j] * y + e[j];
                 ynew[j] = y;
                 double o = y;
-                if (r != 0.) o = is_theta ? y - r * (y - S.P.thb_init[k]) : y - r * y;
+                if (r != 0.) o = is_theta ? y - r * (y - thbi[k]) : y - r * y;
                 f[ix(G, i, k)] = o;
                 if (i == strip1 && publish)
                     lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
@@ -465,8 +485,11 @@ void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t s
     if (S.dslots) {
         const int cw = strip_waves(S);  // compute waves of one column per lane
         const int ns = strips_of(G, cw);
-        const size_t lds = size_t(cw + 1) * G.nz * sizeof(unsigned long long);
-        hipLaunchKernelGGL(k_diff_strips<1>, dim3(ns, NPROG), dim3((cw + 1) * kWave), lds, st, G, S, nudge, ns);
+        const size_t lds = size_t(cw + 4) * G.nz * sizeof(unsigned long long);
+        const dim3 grid(ns, NPROG), block((cw + 1) * kWave);
+        if (cw == 2) hipLaunchKernelGGL((k_diff_strips<1, 2>), grid, block, lds, st, G, S, nudge, ns);
+        else if (cw == 8) hipLaunchKernelGGL((k_diff_strips<1, 8>), grid, block, lds, st, G, S, nudge, ns);
+        else hipLaunchKernelGGL((k_diff_strips<1, 4>), grid, block, lds, st, G, S, nudge, ns);
         return;
     }
     if (n <= kMaxWaves * kWave) launch_e<1>(G, S, nudge, st);

@@ -109,7 +109,7 @@ bool Poisson::learned_guess(hipStream_t st) {
     if (!net || nsolves < 2) return false;
     guess_features(nullptr, st);
     const NetGeo g{1, M, N, 1};
-    if (net->forward(g, gfeat, gdelta, false, st) != VVM_OK) return false;
+    if (net->infer(g, gfeat, gdelta, gscratch, st) != VVM_OK) return false;
     hipLaunchKernelGGL(k_guess_apply, ggrid(N, M + 2), dim3(GX, GY), 0, st, N, M, xprev, xprev2, gdelta, gsc,
                        pcg64.x);
     return true;

@@ -516,6 +516,7 @@ void Poisson::destroy() {
     f = xprev = xprev2 = r64 = nullptr;
     gfeat = gdelta = samples = nullptr;
     gsc = nullptr;
+    gscratch.release();
     sample_cap = sample_count = 0;
     net = nullptr;
     if (hscal) hipHostFree(hscal);

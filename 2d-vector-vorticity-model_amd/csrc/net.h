@@ -54,9 +54,20 @@ void net_loss(long n, const float *out, const float *target, const float *mask, 
 void net_adam(float *w, float *m, float *v, const float *grad, int n, float lr, float b1, float b2, float eps,
               float c1, float c2, hipStream_t st);
 
+// Activation buffers of one inference caller (each timestep handle has its
+// own, so handles that share a network never share activations).
+struct NetScratch {
+    __bf16 *a0 = nullptr, *a1 = nullptr, *z = nullptr;
+    long cap = 0;
+    int reserve(long npx);
+    void release();
+};
+
 // Parameters, optimiser state and activation buffers of one network.
 class GuessNet {
    public:
+    // inference with the caller's buffers: reads only the parameters
+    int infer(const NetGeo &g, const float *feat, float *out, NetScratch &sc, hipStream_t s) const;
     int init(int device, hipStream_t st);
     void destroy();
     // activation buffers for up to npx pixels (grown on demand)

@@ -95,6 +95,45 @@ int GuessNet::reserve(long npx, bool training) {
 
 void GuessNet::repack() { net_pack(params, wf, wb, st); }
 
+int NetScratch::reserve(long npx) {
+    if (npx <= cap) return VVM_OK;
+    release();
+    const size_t e = size_t(npx) * NET_C;
+    a0 = dmalloc<__bf16>(e);
+    a1 = dmalloc<__bf16>(e);
+    z = dmalloc<__bf16>(e);
+    if (!a0 || !a1 || !z) {
+        release();
+        return VVM_ERR_NOMEM;
+    }
+    cap = npx;
+    return VVM_OK;
+}
+
+void NetScratch::release() {
+    for (__bf16 *p : {a0, a1, z})
+        if (p) hipFree(p);
+    a0 = a1 = z = nullptr;
+    cap = 0;
+}
+
+int GuessNet::infer(const NetGeo &g, const float *feat, float *out, NetScratch &sc, hipStream_t s) const {
+    const long npx = long(g.nb) * g.h * g.w;
+    int rc = sc.reserve(npx);
+    if (rc) return rc;
+    __bf16 *buf[2] = {sc.a0, sc.a1};
+    net_pack_input(g, feat, buf[0], s);
+    for (int l = 0; l < NET_L; l++) {
+        const __bf16 *wl = wf + size_t(l) * NET_PACK;
+        const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
+        if (l == NET_L - 1)
+            net_conv_last(g, buf[l & 1], wl, bl, out, s);
+        else
+            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], sc.z, s);
+    }
+    return VVM_OK;
+}
+
 int GuessNet::forward(const NetGeo &g, const float *feat, float *out, bool keep, hipStream_t s) {
     const long npx = long(g.nb) * g.h * g.w;
     int rc = reserve(npx, keep);

@@ -151,6 +151,15 @@ def load_lib(path: str = LIB_PATH):
         "vvm_net_load": (C.c_int, [C.c_void_p, C.c_char_p]),
         "vvm_dev_set_guess_net": (C.c_int, [C.c_void_p, C.c_void_p]),
         "vvm_dev_tune": (C.c_int, [C.c_void_p, C.c_int, P(VvmTuneResult)]),
+        # batched ensemble (include/vvm_ens.h)
+        "vvm_ens_create": (C.c_int, [P(VvmEnsMember), C.c_int, C.c_int, C.c_int, P(C.c_void_p)]),
+        "vvm_ens_destroy": (None, [C.c_void_p]),
+        "vvm_ens_last_error": (C.c_char_p, [C.c_void_p]),
+        "vvm_ens_size": (C.c_int, [C.c_void_p]),
+        "vvm_ens_handle": (C.c_void_p, [C.c_void_p, C.c_int]),
+        "vvm_ens_set_guess_net": (C.c_int, [C.c_void_p, C.c_void_p]),
+        "vvm_ens_tune": (C.c_int, [C.c_void_p, C.c_int]),
+        "vvm_ens_step": (C.c_int, [C.c_void_p, C.c_int, P(VvmEnsStats)]),
         "vvm_dev_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -354,13 +363,15 @@ class VvmError(RuntimeError):
 class Device:
     """One device handle (one GPU, one stream)."""
 
-    def __init__(self, cfg: VvmDevConfig, device: int = 0):
+    def __init__(self, cfg: VvmDevConfig, device: int = 0, handle: int | None = None):
         self.lib = load_lib()
         self.cfg = cfg
-        self.h = C.c_void_p()
-        rc = self.lib.vvm_dev_create(C.byref(cfg), device, C.byref(self.h))
-        if rc != 0:
-            raise VvmError(f"vvm_dev_create failed ({rc})")
+        self.owned = handle is None
+        self.h = C.c_void_p(handle) if handle is not None else C.c_void_p()
+        if self.owned:
+            rc = self.lib.vvm_dev_create(C.byref(cfg), device, C.byref(self.h))
+            if rc != 0:
+                raise VvmError(f"vvm_dev_create failed ({rc})")
 
     def _check(self, rc: int, what: str) -> int:
         if rc < 0:
@@ -427,8 +438,71 @@ class Device:
         return s
 
     def close(self) -> None:
-        if self.h:
+        if self.h and self.owned:
             self.lib.vvm_dev_destroy(self.h)
+        self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class VvmEnsMember(C.Structure):
+    _fields_ = [("cfg", VvmDevConfig), ("params", VvmCaseParams)]
+
+
+class VvmEnsStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("grid_point_steps", C.c_double), ("solve_iters_total", C.c_int64),
+                ("not_converged", C.c_int32), ("members", C.c_int32)]
+
+
+class Ensemble:
+    """Members of uneven grids advanced together on one device
+    (include/vvm_ens.h)."""
+
+    def __init__(self, cases: list, device: int = 0, threads: int = 0):
+        self.lib = load_lib()
+        arr = (VvmEnsMember * len(cases))()
+        for i, c in enumerate(cases):
+            arr[i].cfg = c.cfg
+            arr[i].params = c.params
+        self.cases = list(cases)
+        self.h = C.c_void_p()
+        rc = self.lib.vvm_ens_create(arr, len(cases), device, threads, C.byref(self.h))
+        if rc != 0:
+            raise VvmError(f"vvm_ens_create failed ({rc})")
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            raise VvmError(f"{what} failed ({rc}): {self.lib.vvm_ens_last_error(self.h).decode()}")
+        return rc
+
+    def member(self, i: int) -> Device:
+        h = self.lib.vvm_ens_handle(self.h, i)
+        return Device(self.cases[i].cfg, handle=h)
+
+    def set_guess_net(self, net: "Net | None") -> None:
+        self._check(self.lib.vvm_ens_set_guess_net(self.h, net.h if net is not None else None), "set_guess_net")
+
+    def tune(self, steps: int = 2) -> None:
+        self._check(self.lib.vvm_ens_tune(self.h, steps), "tune")
+
+    def step(self, n: int) -> VvmEnsStats:
+        s = VvmEnsStats()
+        self._check(self.lib.vvm_ens_step(self.h, n, C.byref(s)), "step")
+        return s
+
+    def close(self) -> None:
+        if self.h:
+            self.lib.vvm_ens_destroy(self.h)
             self.h = C.c_void_p()
 
     def __enter__(self):

@@ -112,8 +112,68 @@ def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> None:
     log(f"[bench] copy calibration: {reps} x {nbytes} bytes device-to-device")
 
 
+def ensemble_bench(steps: int, net_path: str | None, seed: int = 4) -> dict:
+    """Config 4: 64 members, nx in {256, 512, 1024, 2048}, nz in {64, 128,
+    256}, dx = dz = 100 m, per-member seed, theta noise sigma U(0.05, 0.2) K,
+    surface heat flux U(50, 200) W/m^2; mixed-precision solve (learned guess
+    when a trained network is given, else the extrapolated guess), tuned
+    launch shapes.  The same ensemble in FP64 (solve to round-off) gives the
+    per-member error."""
+    rng = np.random.default_rng(seed)
+    draws = [(int(rng.choice([256, 512, 1024, 2048])), int(rng.choice([64, 128, 256])),
+              float(rng.uniform(0.05, 0.2)), float(rng.uniform(50, 200))) for _ in range(64)]
+    guess = V.VVM_GUESS_LEARNED if net_path else V.VVM_GUESS_EXTRAP
+
+    def cases(**over):
+        return [V.grid_case(nx, nz, seed=1000 + k, sigma=sg, heat=hf, **over)
+                for k, (nx, nz, sg, hf) in enumerate(draws)]
+
+    fast = cases(stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, solver_mode=V.VVM_SOLVER_MIXED, guess_mode=guess)
+    net = None
+    if net_path:
+        net = V.Net()
+        net.load(net_path)
+    finals = {}
+    out = {}
+    for name, cs in (("mixed", fast), ("fp64", cases(stop_mode=V.VVM_STOP_PARITY))):
+        with V.Ensemble(cs) as ens:
+            if name == "mixed":
+                if net is not None:
+                    ens.set_guess_net(net)
+                ens.tune(2)
+            ens.step(2)  # warm-up
+            log(f"[bench] ensemble {name}: 64 members, {steps} steps")
+            s = ens.step(steps)
+            out[name] = {"value": s.grid_point_steps / s.seconds, "seconds": s.seconds,
+                         "solve_iters_per_member_step": s.solve_iters_total / (64 * steps),
+                         "not_converged": s.not_converged}
+            finals[name] = []
+            for i, c in enumerate(cs):
+                st = V.init_state(c)
+                ens.member(i).download(st)
+                finals[name].append(st)
+    errs = []
+    for a, b in zip(finals["mixed"], finals["fp64"]):
+        errs.append(max(float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300))
+                        for f in ("zeta", "th", "qv", "u", "w")))
+    pts = sum(nx * nz for nx, nz, _, _ in draws)
+    return {
+        "metric": "ensemble grid-point-steps/s (64 uneven members, mixed precision)",
+        "value": out["mixed"]["value"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": steps,
+        "higher_is_better": True, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": "config 4: 64 members, nx in {256,512,1024,2048}, nz in {64,128,256}, "
+                               "dx=dz=100 m", "grid_points_total": pts,
+                   "guess": "learned" if net_path else "extrapolated"},
+        "fp64_value": out["fp64"]["value"], "paths": out,
+        "member_error_max_rel_l2": max(errs), "member_error_median_rel_l2": float(np.median(errs)),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--ensemble", action="store_true",
+                    help="time the 64-member ensemble of uneven grids instead of the 2048x256 step")
+    ap.add_argument("--net", default=None, help="trained guess network (vvm_net_save file) for the mixed path")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +187,9 @@ def main():
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the timestep is single-device; see DESIGN.md for how it would shard)", file=sys.stderr)
         sys.exit(2)
+    if args.ensemble:
+        print(json.dumps(ensemble_bench(args.steps, args.net)), flush=True)
+        return
 
     paths = {
         "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
