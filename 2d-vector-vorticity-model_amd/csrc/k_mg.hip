@@ -242,11 +242,22 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T 
             __syncthreads();
         }
     }
-    {  // coarsest: damped Jacobi from zero
+    {  // coarsest: exact (dense inverse) when small, else damped Jacobi from zero
         const int l = L - 1;
-        for (int it = 0; it < cp.coarse_iters; it += 2) {
-            sweep(l, X(l), Tm);
-            sweep(l, Tm, X(l));
+        const int N = cp.N[l], M = cp.M[l], n = N * M;
+        if (cp.cinv && cp.cinv_n == n) {
+            for (int t = threadIdx.x; t < n; t += blockDim.x) {
+                const T *row = cp.cinv + size_t(t) * n;
+                T s = T(0);
+                for (int q = 0; q < n; q++) s += row[q] * B(l)[N + q];  // rows 1..M start at offset N
+                X(l)[N + t] = s;
+            }
+            __syncthreads();
+        } else {
+            for (int it = 0; it < cp.coarse_iters; it += 2) {
+                sweep(l, X(l), Tm);
+                sweep(l, Tm, X(l));
+            }
         }
     }
     for (int l = L - 2; l >= 0; l--) {
@@ -325,6 +336,8 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
     cp.offT = off;
     cp.Tsize = tmax;
     cp.omega = H.omega;
+    cp.cinv = H.cinv;
+    cp.cinv_n = H.cinv_n;
     cp.coarse_iters = H.coarse_iters;
     H.coarse_lds = size_t(off + tmax) * sizeof(T);
     if (global) {

@@ -370,6 +370,62 @@ bool build_hier(MGHier<T> &H, int N, int M, const std::vector<double> &cx0, cons
         M = Mc;
     }
     H.nlev = l;
+    // exact solve of a small coarsest level: dense inverse of its operator
+    // (SPD: diagonal 2 cx + cu_j + cu_{j-1}, -cx to the periodic x
+    // neighbours, -cu to the z neighbours, Dirichlet rows dropped), by
+    // Gauss-Jordan elimination in FP64
+    {
+        const int n = N * M;
+        if (n > 0 && n <= 512) {
+            std::vector<double> A(size_t(n) * n, 0.), I(size_t(n) * n, 0.);
+            for (int j = 1; j <= M; j++)
+                for (int i = 0; i < N; i++) {
+                    const int t = (j - 1) * N + i;
+                    A[size_t(t) * n + t] += 2. * cx[j] + cu[j] + cu[j - 1];
+                    A[size_t(t) * n + (j - 1) * N + (i + N - 1) % N] -= cx[j];
+                    A[size_t(t) * n + (j - 1) * N + (i + 1) % N] -= cx[j];
+                    if (j + 1 <= M) A[size_t(t) * n + j * N + i] -= cu[j];
+                    if (j - 1 >= 1) A[size_t(t) * n + (j - 2) * N + i] -= cu[j - 1];
+                }
+            for (int t = 0; t < n; t++) I[size_t(t) * n + t] = 1.;
+            bool ok = true;
+            for (int c = 0; c < n && ok; c++) {
+                int piv = c;
+                for (int r = c + 1; r < n; r++)
+                    if (std::fabs(A[size_t(r) * n + c]) > std::fabs(A[size_t(piv) * n + c])) piv = r;
+                if (A[size_t(piv) * n + c] == 0.) {
+                    ok = false;
+                    break;
+                }
+                if (piv != c)
+                    for (int k = 0; k < n; k++) {
+                        std::swap(A[size_t(c) * n + k], A[size_t(piv) * n + k]);
+                        std::swap(I[size_t(c) * n + k], I[size_t(piv) * n + k]);
+                    }
+                const double inv = 1. / A[size_t(c) * n + c];
+                for (int k = 0; k < n; k++) {
+                    A[size_t(c) * n + k] *= inv;
+                    I[size_t(c) * n + k] *= inv;
+                }
+                for (int r = 0; r < n; r++) {
+                    if (r == c) continue;
+                    const double fct = A[size_t(r) * n + c];
+                    if (fct == 0.) continue;
+                    for (int k = 0; k < n; k++) {
+                        A[size_t(r) * n + k] -= fct * A[size_t(c) * n + k];
+                        I[size_t(r) * n + k] -= fct * I[size_t(c) * n + k];
+                    }
+                }
+            }
+            if (ok) {
+                std::vector<T> Ti(I.begin(), I.end());
+                H.cinv = dalloc<T>(size_t(n) * n);
+                if (!H.cinv) return false;
+                hipMemcpy(H.cinv, Ti.data(), sizeof(T) * Ti.size(), hipMemcpyHostToDevice);
+                H.cinv_n = n;
+            }
+        }
+    }
     // the one-workgroup kernel takes every level that fits in 64 KiB of LDS;
     // the finest levels run as tiled down/up kernels
     return mg_plan(H, size_t(64) * 1024) == 0;
@@ -390,6 +446,9 @@ void free_hier(MGHier<T> &H) {
     }
     if (H.cws) hipFree(H.cws);
     H.cws = nullptr;
+    if (H.cinv) hipFree(H.cinv);
+    H.cinv = nullptr;
+    H.cinv_n = 0;
     H.nlev = 0;
 }
 

@@ -25,6 +25,8 @@ struct CoarseParams {
     int offT = 0, Tsize = 0;
     T omega = T(0.8);
     int coarse_iters = 32;
+    const T *cinv = nullptr;  // dense inverse of the coarsest level (or null)
+    int cinv_n = 0;
 };
 
 template <typename T>
@@ -37,6 +39,10 @@ struct MGHier {
     size_t coarse_lds = 0;  // its LDS bytes (0: the levels live in cws)
     T *cws = nullptr;       // global-memory workspace of the one-workgroup
                             // kernel when even the coarsest level exceeds LDS
+    // The coarsest level is solved exactly when it is small: its operator's
+    // dense inverse (row-major, unknown t = (j-1) N + i), built once in FP64.
+    T *cinv = nullptr;
+    int cinv_n = 0;
     CoarseParams<T> cp;
 };
 

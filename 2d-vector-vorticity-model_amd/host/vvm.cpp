@@ -285,6 +285,32 @@ void vvm::Iteration::pzeta_pt(vvm &model) { run_op(model, VVM_OP_PZETA, "pzeta_p
 void vvm::Iteration::pthermo_pt(vvm &model) { run_op(model, VVM_OP_PSCALARS, "pthermo_pt"); }
 void vvm::Iteration::nextTimeStep(vvm &model) { run_op(model, VVM_OP_NEXT, "nextTimeStep"); }
 
+void vvm::Init::Init1d(vvm &model) {
+    model.sync_to_host();
+    vvm_host_view v = model.host_view();
+    const int rc = vvm_init_profiles(&model.dev_config, model.sounding_path.empty() ? nullptr : model.sounding_path.c_str(), &v);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: Init1d failed (%d)%s%s\n", rc, model.sounding_path.empty() ? "" : " reading ",
+                     model.sounding_path.c_str());
+        std::exit(1);
+    }
+    model.ubarTop = v.ubarTop, model.ubarTopp = v.ubarTopp, model.ubarTopm = v.ubarTopm;
+    model.step = v.step;
+    model.dev_stale = true;
+}
+
+void vvm::Init::Init2d(vvm &model) {
+    model.sync_to_host();
+    vvm_host_view v = model.host_view();
+    const int rc = vvm_init_fields(&model.dev_config, &v);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: Init2d failed (%d)\n", rc);
+        std::exit(1);
+    }
+    model.ubarTop = v.ubarTop, model.ubarTopm = v.ubarTopm;
+    model.dev_stale = true;
+}
+
 void vvm::Init::InitCase(vvm &model, const vvm_case_params &p) {
     model.sync_to_host();
     vvm_host_view v = model.host_view();

@@ -126,6 +126,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_init_case": (C.c_int, [P(VvmDevConfig), P(VvmCaseParams), P(VvmHostView)]),
         "vvm_random_perturbation": (C.c_int, [P(VvmDevConfig), P(VvmHostView), C.c_int,
                                               C.c_double, C.c_double, C.c_double]),
+        "vvm_init_reference": (C.c_int, [P(VvmDevConfig), C.c_char_p, P(VvmHostView)]),
         "vvm_field_len": (C.c_int64, [C.c_int, C.c_int]),
         "vvm_hist_len": (C.c_int64, [C.c_int, C.c_int]),
         # learned initial guess (include/vvm_net.h)
@@ -340,6 +341,19 @@ class HostState:
         h.step = int(r["step"][0])
         h.status_for_adding_forcing = int(r["status_for_adding_forcing"][0])
         return h
+
+
+def init_reference(cfg: VvmDevConfig, sounding_path: str | None = None) -> HostState:
+    """The reference's own initial state (Init1d + Init2d): the analytic
+    sounding, or the 10-column sounding file when a path is given."""
+    lib = load_lib()
+    h = HostState(cfg.nx, cfg.nz)
+    v = h.view()
+    rc = lib.vvm_init_reference(C.byref(cfg), sounding_path.encode() if sounding_path else None, C.byref(v))
+    if rc != 0:
+        raise VvmError(f"vvm_init_reference failed: {rc}")
+    h.pull_scalars(v)
+    return h
 
 
 def init_state(c: Case) -> HostState:

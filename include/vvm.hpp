@@ -142,6 +142,7 @@ class vvm {
     bool host_stale = false;
     bool dev_stale = true;
     bool print_steps = true;           // `std::cout << step` each step, as the reference
+    std::string sounding_path;         // LOADINITPATH of the reference; empty: analytic sounding
     long long solve_iters_total = 0;   // PCG iterations over the run
     vvm_host_view host_view();          // pointers into this model's arrays
     void sync_to_device();              // upload when dev_stale
@@ -193,8 +194,14 @@ class vvm {
 
     class Init {
        public:
-        // Base state and initial fields of the idealised cases (warm bubble
-        // over the exponential-theta sounding); see vvm_init_case.
+        // src/Init.cpp:14-135: the analytic sounding, or the sounding file
+        // model.sounding_path when set (LoadFile, src/Init.cpp:302-370).
+        static void Init1d(vvm &model);
+        // src/Init.cpp:137-266: initial fields from the profiles
+        // (TROPICALFORCING branch when the config's tropical_forcing is set).
+        static void Init2d(vvm &model);
+        // Base state and initial fields of the idealised benchmark cases
+        // (warm bubble over the exponential-theta sounding); see vvm_init_case.
         static void InitCase(vvm &model, const vvm_case_params &p);
         static void RandomPerturbation(vvm &model, int t, double min_range = -0.25, double max_range = 0.25,
                                        double standard_deviation = 1.);

@@ -223,6 +223,22 @@ typedef struct vvm_case_params {
 } vvm_case_params;
 int vvm_init_case(const vvm_dev_config *cfg, const vvm_case_params *p, vvm_host_view *v);
 
+/* The reference's own initial state: vvm::Init::Init1d then Init2d
+ * (src/Init.cpp:14-266).  sounding_path NULL selects Init1d's analytic
+ * sounding (src/Init.cpp:17-116); otherwise the 10-column sounding file of
+ * LoadFile (src/Init.cpp:302-370; header line and zero level skipped, the
+ * first nz-2 levels used).  cfg->tropical_forcing selects Init2d's
+ * TROPICALFORCING branch (random theta kick, Q1LS/Q2LS from the file);
+ * otherwise cfg->CASE picks the initial fields (0 mean state, 1 warm
+ * bubble, 2 bubble and shear) and cfg->addforcingtime > 0 adds the kick.
+ * Every array of the view is written (zeroed first, as a new model).
+ * Returns VVM_ERR_SIZE when the file has fewer levels than the grid. */
+int vvm_init_reference(const vvm_dev_config *cfg, const char *sounding_path, vvm_host_view *v);
+/* The two halves: Init1d (zeroes the view, then the base-state profiles)
+ * and Init2d (the initial fields from those profiles). */
+int vvm_init_profiles(const vvm_dev_config *cfg, const char *sounding_path, vvm_host_view *v);
+int vvm_init_fields(const vvm_dev_config *cfg, vvm_host_view *v);
+
 /* The reference's Init::RandomPerturbation (src/Init.cpp:488-515): fills
  * init_th_forcing from std::mt19937(t) and std::normal_distribution. */
 int vvm_random_perturbation(const vvm_dev_config *cfg, vvm_host_view *v, int t,

@@ -30,7 +30,7 @@ mkdir -p "$OUT"
 STAGE="$OUT/stage.$$"
 rm -rf "$STAGE"
 mkdir -p "$STAGE"
-trap 'rm -rf "$STAGE"' EXIT
+trap 'rm -rf "$STAGE" "$STAGE".load "$STAGE".notrop' EXIT
 
 FILES="Config.hpp Declare.hpp Timer.hpp Iteration.cpp Advection.cpp Buoyancy.cpp Boundary.cpp \
 PoissonSolver.cpp Turbulent.cpp MicroPhysics.cpp NumericalProcess.cpp AddForcing.cpp Init.cpp"
@@ -63,3 +63,29 @@ $CXX -O3 -std=c++17 -ffp-contract=off -w -I"$REF/include" -I"$STAGE" \
     $SRCS "$HERE/ref_harness.cpp" -o "$OUT/vvm_ref.tmp"
 mv "$OUT/vvm_ref.tmp" "$OUT/vvm_ref"
 echo "build_ref: built $OUT/vvm_ref"
+
+# Two more configurations of the same sources, for the initial-state
+# fixtures only (mode "init"):
+#   vvm_ref_load   -- LOADFILE on, LOADINITPATH = the reference's own
+#                     input/init.txt (Init1d reads the sounding file);
+#   vvm_ref_notrop -- TROPICALFORCING off (Init2d's CASE branches).
+variant() {
+    local name=$1
+    shift
+    local V="$STAGE.$name"
+    rm -rf "$V"
+    cp -r "$STAGE" "$V"
+    for edit in "$@"; do sed -i "$edit" "$V/Config.hpp"; done
+    local VS=""
+    for f in $FILES; do case $f in *.cpp) VS="$VS $V/$f";; esac; done
+    $CXX -O3 -std=c++17 -ffp-contract=off -w -I"$REF/include" -I"$V" \
+        $VS "$HERE/ref_harness.cpp" -o "$OUT/vvm_ref_$name.tmp"
+    mv "$OUT/vvm_ref_$name.tmp" "$OUT/vvm_ref_$name"
+    rm -rf "$V"
+    echo "build_ref: built $OUT/vvm_ref_$name"
+}
+grep -q '^    #define LOADINITPATH' "$cfg" || { echo "build_ref: LOADINITPATH line not found" >&2; exit 1; }
+variant load "s|^// #define LOADFILE\b|#define LOADFILE|" \
+    "0,/^    #define LOADINITPATH .*/s||    #define LOADINITPATH \"$REF/input/init.txt\"|"
+grep -q '^#define TROPICALFORCING\b' "$cfg" || { echo "build_ref: TROPICALFORCING line not found" >&2; exit 1; }
+variant notrop "s|^#define TROPICALFORCING\b|// #define TROPICALFORCING|"

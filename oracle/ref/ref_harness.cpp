@@ -6,6 +6,7 @@
 //   vvm_ref step <nsteps> <in.state> <out.state>     TimeMarching (src/Iteration.cpp:208-438)
 //   vvm_ref op <name> <in.state> <out.state>         one reference operator
 //   vvm_ref solve <tol> <in.state> <out.state>       cal_w's Eigen solve at a given tolerance
+//   vvm_ref init - <in.state> <out.state>            Init1d + Init2d on the grid of in.state
 //
 // State file: "VVMSTATE", int32 version, int32 nrec, then records
 //   { char name[32]; int32 ndim; int32 dims[3]; double data[prod(dims)] }.
@@ -125,7 +126,9 @@ std::vector<Binding> bindings(vvm &m) {
         {"thvb", m.thvb, nz, 1, 1},     {"thvbm", m.thvbm, nz, 1, 1}, {"qvb", m.qvb, nz, 1, 1},
         {"qvb0", m.qvb0, nz, 1, 1},     {"qvsb", m.qvsb, nz, 1, 1},   {"z", m.z, nz, 1, 1},
         {"z_zeta", m.z_zeta, nz, 1, 1}, {"lambda2", m.lambda2, nz, 1, 1},
+#if defined(TROPICALFORCING)
         {"Q1LS", m.Q1LS, nz, 1, 1},     {"Q2LS", m.Q2LS, nz, 1, 1},
+#endif
     };
 }
 
@@ -190,6 +193,16 @@ int main(int argc, char **argv) {
     if (model.nx != nx || model.nz != nz) {
         std::fprintf(stderr, "grid mismatch: model %dx%d state %dx%d\n", model.nx, model.nz, nx, nz);
         return 2;
+    }
+    if (mode == "init") {
+        // the reference's own initial state (Init1d + Init2d as main() calls
+        // them, src/main.cpp:78-83); the input file supplies the grid only
+        vvm::Init::Init1d(model);
+        vvm::Init::Init2d(model);
+        Writer w;
+        save_from(model, w);
+        w.save(argv[4]);
+        return 0;
     }
     load_into(model, recs);
     vvm::PoissonSolver::InitPoissonMatrix(model);

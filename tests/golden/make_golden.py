@@ -130,12 +130,47 @@ def long_fixture(tmp):
     np.savez_compressed(os.path.join(HERE, "long_c0.npz"), **data)
 
 
+INIT_VARIANTS = {
+    # name: (binary, nx_int, nz_int, dx, CASE, addforcingtime, tropical)
+    "analytic_trop": ("vvm_ref", 32, 40, 200.0, 1, 0.0, 1),
+    "load_trop": ("vvm_ref_load", 32, 100, 200.0, 1, 0.0, 1),
+    "notrop_case0": ("vvm_ref_notrop", 32, 40, 200.0, 0, 0.0, 0),
+    "notrop_case1": ("vvm_ref_notrop", 48, 40, 200.0, 1, 0.0, 0),
+    "notrop_case2": ("vvm_ref_notrop", 48, 40, 200.0, 2, 0.0, 0),
+    "notrop_case1_kick": ("vvm_ref_notrop", 32, 40, 200.0, 1, 10.0, 0),
+}
+INIT_FIELDS = ["th", "thm", "qv", "qvm", "u", "zeta", "zetam", "init_th_forcing"]
+
+
+def init_fixture(tmp):
+    """The reference's own Init1d + Init2d (analytic sounding, sounding file
+    input/init.txt, TROPICALFORCING on and off, CASE 0/1/2, the random kick)."""
+    data = {}
+    for name, (binary, nxi, nzi, dx, case_, aft, trop) in INIT_VARIANTS.items():
+        cfg = V.make_config(nxi, nzi, dx, dx, 2.0, CASE=case_, addforcingtime=aft, tropical_forcing=trop)
+        st = V.HostState(cfg.nx, cfg.nz)
+        pin, pout = os.path.join(tmp, "init_in.state"), os.path.join(tmp, "init_out.state")
+        st.save(pin, cfg)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", binary), "init", "-", pin, pout], check=True, env=env,
+                       timeout=120)
+        r = V.HostState.read_records(pout)
+        for f in INIT_FIELDS:
+            data[f"{name}/{f}"] = r[f]
+        for f in V.PROFILES:
+            if f in r:
+                data[f"{name}/{f}"] = r[f]
+        for f in ("ubarTop", "ubarTopm"):
+            data[f"{name}/{f}"] = r[f]
+    np.savez_compressed(os.path.join(HERE, "init_ref.npz"), **data)
+
+
 def main():
     if not O.ref_available():
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "ref", "build_ref.sh")], check=True)
     if not O.ref_available():
         sys.exit("reference binary unavailable")
-    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long"}
+    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long", "init"}
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as tmp:
         if "ops" in which:
             ops_fixture(tmp)
@@ -145,6 +180,8 @@ def main():
             misc_fixture(tmp)
         if "long" in which:
             long_fixture(tmp)
+        if "init" in which:
+            init_fixture(tmp)
 
 
 if __name__ == "__main__":
