@@ -53,6 +53,36 @@ struct Affine {
 // g after f: y = g.a (f.a x + f.b) + g.b
 __device__ inline Affine after(const Affine &g, const Affine &f) { return {g.a * f.a, g.a * f.b + g.b}; }
 
+// A double moved between lanes by a DPP control (two 32-bit moves); lanes
+// whose source is outside the pattern, or outside row_mask, keep `old`.
+template <int CTRL, int ROWS>
+__device__ inline double dppd(double old, double v) {
+    const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(int(o & 0xffffffffll), int(x & 0xffffffffll), CTRL, ROWS, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(int(o >> 32), int(x >> 32), CTRL, ROWS, 0xF, false);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// Inclusive scan of affine maps over the 64 lanes, inc_l = f_l o ... o f_0,
+// on DPP moves (a few cycles each, no LDS round trip): row_shr 1, 2, 4, 8
+// inside each 16-lane row, then row_bcast 15 (rows 1, 3) and row_bcast 31
+// (rows 2, 3) across rows.  Lanes without a source combine with the
+// identity map.
+template <int CTRL, int ROWS>
+__device__ inline void scan_step(Affine &f) {
+    const Affine p{dppd<CTRL, ROWS>(1., f.a), dppd<CTRL, ROWS>(0., f.b)};
+    f = after(f, p);
+}
+__device__ inline Affine wave_scan(Affine f) {
+    scan_step<0x111, 0xF>(f);  // row_shr:1
+    scan_step<0x112, 0xF>(f);  // row_shr:2
+    scan_step<0x114, 0xF>(f);  // row_shr:4
+    scan_step<0x118, 0xF>(f);  // row_shr:8
+    scan_step<0x142, 0xA>(f);  // row_bcast:15 into rows 1 and 3
+    scan_step<0x143, 0xC>(f);  // row_bcast:31 into rows 2 and 3
+    return f;
+}
+
 // a, c, d of every interior point (Turbulent.cpp:81-86 for Mparam on zeta,
 // :105-110 for Hparam on th, qv, qc, qr).  Row k = 1 folds the (old) ghost
 // row below into c.
@@ -377,13 +407,9 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
                 if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
             }
             load_row<E>(G, A_, C_, D_, i0, n, k + PFS, nz, ba[q], bc[q], bd[q]);
-            Affine inc = m;
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const double pa = __shfl_up(inc.a, off, kWave), pb = __shfl_up(inc.b, off, kWave);
-                if (lane >= off) inc = after(inc, Affine{pa, pb});
-            }
-            const double ea = __shfl_up(inc.a, 1, kWave), eb = __shfl_up(inc.b, 1, kWave);
+            const Affine inc = wave_scan(m);
+            // exclusive prefix: the inclusive one of the lane below (identity at lane 0)
+            const double ea = dppd<0x138, 0xF>(1., inc.a), eb = dppd<0x138, 0xF>(0., inc.b);
             double y0;
             if (!src) {
                 y0 = ghost[k];  // ghost column, old as in the reference
@@ -399,7 +425,10 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
                         __builtin_amdgcn_s_sleep(1);
                     }
                 }
-                y0 = __shfl(__longlong_as_double(static_cast<long long>(bits)), 0, kWave);
+                // lane 0 is the first active lane: a scalar broadcast
+                const int lo = __builtin_amdgcn_readfirstlane(int(bits & 0xffffffffull));
+                const int hi = __builtin_amdgcn_readfirstlane(int(bits >> 32));
+                y0 = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
             }
             double y = (lane == 0) ? y0 : ea * y0 + eb;
             const double r = rate_l[k];

@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per interior point and launch: distinct 2-D arrays
 # read + written (FP64), 1-D profiles and halo traffic excluded.
 KERNEL_BYTES_PER_POINT = {
-    "diffusion": 96.0,   # 5 fields read+write, RKM, RKH read
+    "diffusion": 120.0,  # Mparam (RKM read, zeta read+write) 24 + Hparam x4 (RKH read, field read+write) 96
     "thermo4": 160.0,    # 4 now + u, w + 5 old AB2 slots read; 4 fut + 5 new slots written
     "pzeta": 96.0,       # zeta, u, w, th, qv, qc, qr + 2 old slots read; zetap + 2 new slots written
     "kessler": 96.0,     # th, qv, qc, qr read and written; 4 process diagnostics written
@@ -47,13 +47,44 @@ KERNEL_BYTES_PER_POINT = {
 }
 
 
+# kernels behind each timed phase (rows of the committed traffic table)
+KERNEL_NAMES = {
+    "diffusion": ["k_diff_coeffs", "k_diff_strips"], "thermo4": ["k_thermo4"], "pzeta": ["k_pzeta"],
+    "kessler": ["k_kessler"], "scale": ["k_scale_forcing"], "turb": ["k_turb_coeffs"], "mean": ["k_update_mean"],
+    "calu": ["k_calu_top", "k_calu_cols"],
+}
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic_2048x256.md")
+
+
+def measured_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch of a timed phase: the sum over its kernels of the
+    calibrated FETCH_SIZE + WRITE_SIZE figures in the committed rocprofv3
+    table (None if the table lacks a kernel)."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    rows = {}
+    for line in open(TRAFFIC_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0].startswith("`"):
+            try:
+                rows[cells[0].strip("`").split("<")[0]] = float(cells[4])
+            except ValueError:
+                pass
+    names = KERNEL_NAMES.get(kernel, [])
+    if not names or any(n not in rows for n in names):
+        return None
+    return sum(rows[n] for n in names)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_path(case: V.Case, state: V.HostState, steps: int, warmup: int, kernel: str | None):
+def run_path(case: V.Case, state: V.HostState, steps: int, warmup: int, kernel: str | None, net=None):
     with V.Device(case.cfg) as dev:
         dev.upload(state)
+        if net is not None:
+            dev.set_guess_net(net)
         if warmup:
             dev.step(warmup)
         if kernel:
@@ -193,7 +224,8 @@ def main():
 
     paths = {
         "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
-        "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+        "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED,
+                      guess_mode=V.VVM_GUESS_LEARNED if args.net else V.VVM_GUESS_EXTRAP),
     }
     wanted = [p for p in args.paths.split(",") if p]
     if args.calibrate_copy:
@@ -205,7 +237,11 @@ def main():
     for name in wanted:
         c = V.case("c2", stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, **paths[name])
         log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed steps of 2048x256")
-        wall, stats, kavg, kn, final = run_path(c, state, args.steps, args.warmup, args.kernel)
+        net = None
+        if args.net and name == "mixed":
+            net = V.Net()
+            net.load(args.net)
+        wall, stats, kavg, kn, final = run_path(c, state, args.steps, args.warmup, args.kernel, net)
         finals[name] = final
         results[name] = {
             "value": pts * args.steps / wall, "ms_per_step": 1e3 * wall / args.steps,
@@ -235,7 +271,8 @@ def main():
         "paths": results,
         "roofline": {"kernel": args.kernel, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "bytes_per_launch": kbytes, "avg_launch_us": R["kernel_avg_us"], "traffic": None},
+                     "bytes_per_launch": kbytes, "avg_launch_us": R["kernel_avg_us"],
+                     "traffic": measured_traffic(args.kernel), "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT)},
     }
     if "fp64_plain" in results and "mixed" in results:
         out["speedup_mixed_vs_fp64"] = results["mixed"]["value"] / results["fp64_plain"]["value"]
