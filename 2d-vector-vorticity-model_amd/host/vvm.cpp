@@ -336,9 +336,113 @@ void vvm::Init::RandomPerturbation(vvm &model, int t, double min_range, double m
     model.dev_stale = true;
 }
 
+namespace {
+
+// netCDF classic (CDF-1) writer: big-endian header of dimensions and
+// variables without attributes, then each variable's doubles at its offset.
+struct NcVar {
+    std::string name;
+    std::vector<int> dims;  // dimension ids
+    const double *data;
+    size_t count;
+};
+
+void put_be32(std::vector<unsigned char> &b, uint32_t v) {
+    for (int s = 24; s >= 0; s -= 8) b.push_back(static_cast<unsigned char>(v >> s));
+}
+
+void put_name(std::vector<unsigned char> &b, const std::string &s) {
+    put_be32(b, uint32_t(s.size()));
+    b.insert(b.end(), s.begin(), s.end());
+    while (b.size() % 4) b.push_back(0);
+}
+
+bool write_nc_classic(const std::string &path, const std::vector<std::pair<std::string, int>> &dims,
+                      const std::vector<NcVar> &vars) {
+    const uint32_t NC_DIMENSION = 10, NC_VARIABLE = 11, NC_DOUBLE = 6;
+    // header size first (offsets depend on it)
+    auto header = [&](const std::vector<uint32_t> &begins) {
+        std::vector<unsigned char> h = {'C', 'D', 'F', 1};
+        put_be32(h, 0);  // numrecs
+        put_be32(h, NC_DIMENSION);
+        put_be32(h, uint32_t(dims.size()));
+        for (auto &d : dims) {
+            put_name(h, d.first);
+            put_be32(h, uint32_t(d.second));
+        }
+        put_be32(h, 0), put_be32(h, 0);  // no global attributes
+        put_be32(h, NC_VARIABLE);
+        put_be32(h, uint32_t(vars.size()));
+        for (size_t v = 0; v < vars.size(); v++) {
+            put_name(h, vars[v].name);
+            put_be32(h, uint32_t(vars[v].dims.size()));
+            for (int id : vars[v].dims) put_be32(h, uint32_t(id));
+            put_be32(h, 0), put_be32(h, 0);  // no attributes
+            put_be32(h, NC_DOUBLE);
+            put_be32(h, uint32_t(vars[v].count * 8));
+            put_be32(h, begins.empty() ? 0u : begins[v]);
+        }
+        return h;
+    };
+    const size_t hsize = header({}).size();
+    std::vector<uint32_t> begins;
+    size_t off = hsize;
+    for (auto &v : vars) {
+        if (off > 0xffffffffull) return false;  // CDF-1 offsets are 32-bit
+        begins.push_back(uint32_t(off));
+        off += v.count * 8;
+    }
+    std::vector<unsigned char> h = header(begins);
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
+    std::vector<unsigned char> buf;
+    for (auto &v : vars) {
+        buf.resize(v.count * 8);
+        for (size_t i = 0; i < v.count; i++) {
+            uint64_t u;
+            std::memcpy(&u, &v.data[i], 8);
+            for (int b = 0; b < 8; b++) buf[i * 8 + b] = static_cast<unsigned char>(u >> (56 - 8 * b));
+        }
+        ok = ok && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    ok = (std::fclose(f) == 0) && ok;
+    return ok;
+}
+
+}  // namespace
+
+void vvm::Output::output_nc(int n, vvm &model) {
+    model.sync_to_host();
+    const size_t f2 = size_t(model.nx) * model.nz;
+    const std::vector<std::pair<std::string, int>> dims = {{"x", model.nx}, {"z", model.nz}};
+    const std::vector<int> xz = {0, 1}, x = {0};
+    // the reference stores ubarTopp in "ubarTop" (src/Output.cpp:87)
+    const std::vector<NcVar> vars = {
+        {"th", xz, model.thcont, f2},
+        {"zeta", xz, model.zetacont, f2},
+        {"u", xz, model.ucont, f2},
+        {"w", xz, model.wcont, f2},
+        {"ubarTop", {}, &model.ubarTopp, 1},
+        {"qv", xz, model.qvcont, f2},
+        {"qc", xz, model.qccont, f2},
+        {"qr", xz, model.qrcont, f2},
+        {"accretion", xz, model.accretioncont, f2},
+        {"autoconversion", xz, model.autoconversioncont, f2},
+        {"evaporation", xz, model.evaporationcont, f2},
+        {"condensation", xz, model.condensationcont, f2},
+        {"precip", x, model.precip, size_t(model.nx)},
+    };
+    const std::string path = model.outputpath + "/nc/" + std::to_string(n) + ".nc";
+    if (!write_nc_classic(path, dims, vars)) {
+        std::fprintf(stderr, "vvm: cannot write %s\n", path.c_str());
+        std::exit(1);
+    }
+}
+
 void vvm::Output::create_all_directory(vvm &model) {
     if (model.outputpath.empty()) return;
-    std::string dirs[] = {model.outputpath, model.outputpath + "/state"};
+    std::string dirs[] = {model.outputpath, model.outputpath + "/state", model.outputpath + "/nc"};
     for (const auto &d : dirs)
         if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) {
             std::fprintf(stderr, "vvm: cannot create %s\n", d.c_str());
@@ -436,8 +540,10 @@ void vvm::Iteration::TimeMarching(vvm &model) {
     const int nmax = int(model.TIMEEND / model.dt);
     ensure_device(model);
     while (model.step < nmax) {
-        if (model.OUTPUTSTEP > 0 && is_output_step(model, model.step) && !model.outputpath.empty())
-            vvm::Output::output_state(model.step, model);
+        if (model.OUTPUTSTEP > 0 && is_output_step(model, model.step) && !model.outputpath.empty()) {
+            if (model.output_nc_files) vvm::Output::output_nc(model.step, model);
+            if (model.output_state_files) vvm::Output::output_state(model.step, model);
+        }
         int next = model.step + 1;
         while (next < nmax && !(model.OUTPUTSTEP > 0 && is_output_step(model, next))) next++;
         const int n = next - model.step;

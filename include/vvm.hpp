@@ -143,6 +143,8 @@ class vvm {
     bool dev_stale = true;
     bool print_steps = true;           // `std::cout << step` each step, as the reference
     std::string sounding_path;         // LOADINITPATH of the reference; empty: analytic sounding
+    bool output_nc_files = true;       // TimeMarching writes nc/<step>.nc on output steps
+    bool output_state_files = false;   // ... and state/state_<step>.bin (exact restart)
     long long solve_iters_total = 0;   // PCG iterations over the run
     vvm_host_view host_view();          // pointers into this model's arrays
     void sync_to_device();              // upload when dev_stale
@@ -209,6 +211,12 @@ class vvm {
 
     class Output {
        public:
+        // <outputpath>/nc/<n>.nc as the reference's per-step output_nc
+        // (src/Output.cpp:49-104): netCDF classic, dimensions x = nx and
+        // z = nz, th, zeta, u, w, qv, qc, qr and the four Kessler process
+        // rates as double [x][z], precip [x], and ubarTop (the reference
+        // writes ubarTopp under that name).
+        static void output_nc(int n, vvm &model);
         static void create_all_directory(vvm &model);
         // The model state at `step` as a VVMSTATE record file
         // <outputpath>/state/state_<step>.bin.

@@ -79,7 +79,7 @@ def test_time_marching_matches_oracle(tmp_path, oracle):
     out = tmp_path / "out"
     final = tmp_path / "final.bin"
     write_config(cfg, VVM_TIMEEND=20, VVM_OUTPUTSTEP=5, VVM_OUTPUTPATH=str(out), VVM_FINAL_STATE=str(final),
-                 **C0_KEYS)
+                 VVM_OUTPUT_STATE=1, **C0_KEYS)
     print(f"vvm_main c0 10 steps -> {final}", flush=True)
     r = subprocess.run([MAIN, str(cfg)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -95,3 +95,67 @@ def test_time_marching_matches_oracle(tmp_path, oracle):
     want = s0.copy()
     oracle.step(c.cfg, want, 10)
     assert_fields_close(got, want, ["zeta", "th", "qv", "qc", "qr", "u", "w"], 1e-9, where="TimeMarching c0")
+    # the reference-format netCDF files of the same output steps
+    from scipy.io import netcdf_file
+    with netcdf_file(str(out / "nc" / "5.nc"), "r", mmap=False) as nc:
+        for f in ("th", "zeta", "u", "w", "qv", "qc", "qr"):
+            np.testing.assert_array_equal(nc.variables[f].data, s5[f], err_msg=f)
+        assert float(nc.variables["ubarTop"].data) == s5.s["ubarTopp"]
+
+
+OUTPUT_PROG = r'''
+#include "vvm.hpp"
+#include <cstdio>
+int main(int argc, char **argv) {
+    // 6 x 4 interior at 200 m; arrays filled with a pattern, no device used
+    Config_VVM c(2., 200., 200., 1200, 800, 2., 10, argv[1], 1, 10., 10., 0.01, 1e-22, 9.80665, 1003.5, 716.5,
+                 287., 2.5e6, 1e5, 96500., 0., 1, 0., 2025, 3, 21, 12., 0., 0., 0., 0.);
+    vvm m(c);
+    for (int i = 0; i < m.nx; i++)
+        for (int k = 0; k < m.nz; k++) {
+            m.th[i][k] = 300. + i + 0.01 * k;
+            m.zeta[i][k] = 1e-3 * i - 2e-4 * k;
+            m.u[i][k] = i * k * 0.5;
+            m.w[i][k] = -0.25 * k;
+            m.qv[i][k] = 1e-3 * (k + 1);
+            m.qc[i][k] = 1e-5 * i;
+            m.qr[i][k] = 1e-6 * (i + k);
+            m.accretion[i][k] = 1.5e-9 * i;
+            m.autoconversion[i][k] = 2.5e-9 * k;
+            m.evaporation[i][k] = -1e-9;
+            m.condensation[i][k] = 3e-9 * (i - k);
+        }
+    for (int i = 0; i < m.nx; i++) m.precip[i] = 0.1 * i;
+    m.ubarTopp = 1.25;
+    m.ubarTop = -7.;
+    vvm::Output::create_all_directory(m);
+    vvm::Output::output_nc(7, m);
+    return 0;
+}
+'''
+
+
+def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
+    """Output::output_nc writes the reference's per-step netCDF layout
+    (src/Output.cpp:49-104); a netCDF-3 reader (scipy) reads it back."""
+    built()
+    src = tmp_path / "out.cpp"
+    src.write_text(OUTPUT_PROG)
+    exe = tmp_path / "out"
+    root = os.path.dirname(PKG)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root}/include", str(src), "-o", str(exe), f"-L{PKG}",
+                        "-lvvm", f"-Wl,-rpath,{PKG}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(tmp_path / "run")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    from scipy.io import netcdf_file
+    with netcdf_file(str(tmp_path / "run" / "nc" / "7.nc"), "r", mmap=False) as nc:
+        assert nc.dimensions == {"x": 8, "z": 6}
+        assert list(nc.variables) == ["th", "zeta", "u", "w", "ubarTop", "qv", "qc", "qr", "accretion",
+                                      "autoconversion", "evaporation", "condensation", "precip"]
+        i, k = np.meshgrid(np.arange(8), np.arange(6), indexing="ij")
+        np.testing.assert_array_equal(nc.variables["th"].data, 300.0 + i + 0.01 * k)
+        np.testing.assert_array_equal(nc.variables["u"].data, i * k * 0.5)
+        np.testing.assert_array_equal(nc.variables["condensation"].data, 3e-9 * (i - k))
+        np.testing.assert_array_equal(nc.variables["precip"].data, 0.1 * np.arange(8))
+        assert float(nc.variables["ubarTop"].data) == 1.25  # ubarTopp, as the reference writes it
