@@ -311,6 +311,20 @@ void vvm::Init::Init2d(vvm &model) {
     model.dev_stale = true;
 }
 
+void vvm::Init::LoadFromPreviousFile(vvm &model) {
+    model.sync_to_host();
+    vvm_host_view v = model.host_view();
+    const int rc = vvm_init_from_previous(&model.dev_config, model.sounding_path.c_str(),
+                                          model.restart_prev_nc.c_str(), model.restart_now_nc.c_str(), &v);
+    if (rc < 0) {
+        std::fprintf(stderr, "vvm: LoadFromPreviousFile failed (%d): %s, %s, %s\n", rc, model.sounding_path.c_str(),
+                     model.restart_prev_nc.c_str(), model.restart_now_nc.c_str());
+        std::exit(1);
+    }
+    model.ubarTopm = v.ubarTopm;
+    model.dev_stale = true;
+}
+
 void vvm::Init::InitCase(vvm &model, const vvm_case_params &p) {
     model.sync_to_host();
     vvm_host_view v = model.host_view();
@@ -535,7 +549,7 @@ void vvm::Output::write_state(const std::string &path, vvm &model) {
 // timers the device time of its last step.
 void vvm::Iteration::TimeMarching(vvm &model) {
     model.sync_to_host();
-    model.step = 0;
+    model.step = model.start_step;  // 0, or TIMENOW after a restart (src/Iteration.cpp:212-219)
     model.dev_stale = true;
     const int nmax = int(model.TIMEEND / model.dt);
     ensure_device(model);

@@ -10,9 +10,11 @@
 //   VVM_DAY VVM_HOUR VVM_MINUTE VVM_SECOND VVM_LON VVM_LAT
 // and the constants main() passes to Config_VVM (src/main.cpp:68-72) are
 // the defaults.  Further keys choose the initial state and the back-end:
-//   VVM_INIT (case|reference): the benchmark cases below, or the
-//     reference's own Init1d/Init2d, with VVM_SOUNDING_FILE (the
-//     LOADINITPATH sounding; empty for the analytic sounding)
+//   VVM_INIT (case|reference|previous): the benchmark cases below, the
+//     reference's own Init1d/Init2d with VVM_SOUNDING_FILE (the
+//     LOADINITPATH sounding; empty for the analytic sounding), or its
+//     LoadFromPreviousFile restart with VVM_RESTART_PREV, VVM_RESTART_NOW
+//     (LOADPATH1/2: two consecutive nc outputs) and VVM_TIMENOW
 //   VVM_ADDFORCINGTIME (10, as main() passes), VVM_TROPICAL_FORCING (0|1)
 //   VVM_BRUNT VVM_BUBBLE_DT VVM_BUBBLE_RADIUS VVM_BUBBLE_ZC
 //   VVM_NOISE (none|uniform|gaussian) VVM_NOISE_AMP VVM_NOISE_ZTOP VVM_SEED
@@ -144,7 +146,14 @@ int main(int argc, char **argv) {
                 model.dt, int(model.TIMEEND / model.dt), config.solver_mode == VVM_SOLVER_MIXED ? "mixed" : "fp64");
     if (dry) return 0;
 
-    if (K.s("VVM_INIT", "case") == "reference") {
+    if (K.s("VVM_INIT", "case") == "previous") {
+        // restart from two outputs of an earlier run (src/main.cpp:75-76)
+        model.sounding_path = K.s("VVM_SOUNDING_FILE", "");
+        model.restart_prev_nc = K.s("VVM_RESTART_PREV", "");
+        model.restart_now_nc = K.s("VVM_RESTART_NOW", "");
+        model.start_step = K.i("VVM_TIMENOW", 1);
+        vvm::Init::LoadFromPreviousFile(model);
+    } else if (K.s("VVM_INIT", "case") == "reference") {
         // the reference's own Init1d / Init2d (src/main.cpp:78-83)
         model.sounding_path = K.s("VVM_SOUNDING_FILE", "");
         vvm::Init::Init1d(model);

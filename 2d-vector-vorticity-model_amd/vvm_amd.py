@@ -127,6 +127,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_random_perturbation": (C.c_int, [P(VvmDevConfig), P(VvmHostView), C.c_int,
                                               C.c_double, C.c_double, C.c_double]),
         "vvm_init_reference": (C.c_int, [P(VvmDevConfig), C.c_char_p, P(VvmHostView)]),
+        "vvm_init_from_previous": (C.c_int, [P(VvmDevConfig), C.c_char_p, C.c_char_p, C.c_char_p, P(VvmHostView)]),
         "vvm_field_len": (C.c_int64, [C.c_int, C.c_int]),
         "vvm_hist_len": (C.c_int64, [C.c_int, C.c_int]),
         # learned initial guess (include/vvm_net.h)
@@ -352,6 +353,19 @@ def init_reference(cfg: VvmDevConfig, sounding_path: str | None = None) -> HostS
     rc = lib.vvm_init_reference(C.byref(cfg), sounding_path.encode() if sounding_path else None, C.byref(v))
     if rc != 0:
         raise VvmError(f"vvm_init_reference failed: {rc}")
+    h.pull_scalars(v)
+    return h
+
+
+def init_from_previous(cfg: VvmDevConfig, sounding_path: str, nc_prev: str, nc_now: str) -> HostState:
+    """The reference's restart from two consecutive netCDF outputs."""
+    lib = load_lib()
+    h = HostState(cfg.nx, cfg.nz)
+    v = h.view()
+    rc = lib.vvm_init_from_previous(C.byref(cfg), sounding_path.encode(), nc_prev.encode(), nc_now.encode(),
+                                    C.byref(v))
+    if rc != 0:
+        raise VvmError(f"vvm_init_from_previous failed: {rc}")
     h.pull_scalars(v)
     return h
 

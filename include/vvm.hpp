@@ -143,6 +143,8 @@ class vvm {
     bool dev_stale = true;
     bool print_steps = true;           // `std::cout << step` each step, as the reference
     std::string sounding_path;         // LOADINITPATH of the reference; empty: analytic sounding
+    std::string restart_prev_nc, restart_now_nc;  // LOADPATH1 / LOADPATH2 of the reference
+    int start_step = 0;                // TIMENOW of the reference (TimeMarching's first step)
     bool output_nc_files = true;       // TimeMarching writes nc/<step>.nc on output steps
     bool output_state_files = false;   // ... and state/state_<step>.bin (exact restart)
     long long solve_iters_total = 0;   // PCG iterations over the run
@@ -202,6 +204,11 @@ class vvm {
         // src/Init.cpp:137-266: initial fields from the profiles
         // (TROPICALFORCING branch when the config's tropical_forcing is set).
         static void Init2d(vvm &model);
+        // src/Init.cpp:372-485: restart from model.restart_prev_nc and
+        // model.restart_now_nc (two consecutive output_nc files) and the
+        // sounding file model.sounding_path; TimeMarching then starts at
+        // model.start_step (the reference's TIMENOW).
+        static void LoadFromPreviousFile(vvm &model);
         // Base state and initial fields of the idealised benchmark cases
         // (warm bubble over the exponential-theta sounding); see vvm_init_case.
         static void InitCase(vvm &model, const vvm_case_params &p);

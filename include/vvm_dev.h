@@ -239,6 +239,15 @@ int vvm_init_reference(const vvm_dev_config *cfg, const char *sounding_path, vvm
 int vvm_init_profiles(const vvm_dev_config *cfg, const char *sounding_path, vvm_host_view *v);
 int vvm_init_fields(const vvm_dev_config *cfg, vvm_host_view *v);
 
+/* The reference's Init::LoadFromPreviousFile (src/Init.cpp:372-485), the
+ * restart from two consecutive per-step netCDF outputs (nc_prev: the step
+ * before, nc_now: the restart step; as written by output_nc) and the
+ * sounding file.  Writes what the reference writes on top of the view
+ * (which should be zero, as a new model); other arrays are left alone, as
+ * the reference leaves them.  Reads netCDF classic files (CDF-1 / CDF-2). */
+int vvm_init_from_previous(const vvm_dev_config *cfg, const char *sounding_path, const char *nc_prev,
+                           const char *nc_now, vvm_host_view *v);
+
 /* The reference's Init::RandomPerturbation (src/Init.cpp:488-515): fills
  * init_th_forcing from std::mt19937(t) and std::normal_distribution. */
 int vvm_random_perturbation(const vvm_dev_config *cfg, vvm_host_view *v, int t,

@@ -50,6 +50,64 @@ def test_init_short_sounding_file_is_rejected(tmp_path):
         V.init_reference(cfg, str(short))
 
 
+def _write_nc(path, fields, ubar):
+    """A per-step output as the reference's output_nc writes it, through an
+    independent netCDF-3 writer (scipy)."""
+    from scipy.io import netcdf_file
+    nx, nz = fields["th"].shape
+    with netcdf_file(str(path), "w") as nc:
+        nc.createDimension("x", nx)
+        nc.createDimension("z", nz)
+        for f, a in fields.items():
+            var = nc.createVariable(f, "d", ("x", "z"))
+            var[:] = a
+        var = nc.createVariable("ubarTop", "d", ())
+        var.data[...] = ubar
+
+
+@pytest.mark.parametrize("case_", [0, 1])
+def test_restart_from_previous_outputs(tmp_path, case_):
+    """LoadFromPreviousFile (src/Init.cpp:372-485) reads two outputs and the
+    sounding; the fields, the bubble added back for CASE 1, the halos and the
+    recomputed horizontal means follow the reference's statements."""
+    nxi, nzi = 16, 100
+    cfg = V.make_config(nxi, nzi, 200.0, 200.0, 2.0, CASE=case_, tropical_forcing=1)
+    nx, nz = cfg.nx, cfg.nz
+    rng = np.random.default_rng(case_)
+    prev = {f: rng.normal(0, 1, (nx, nz)) for f in ("th", "zeta", "qv", "qc", "qr", "u", "w")}
+    now = {f: rng.normal(0, 1, (nx, nz)) for f in ("th", "zeta", "qv", "qc", "qr", "u", "w")}
+    now["th"] += 300.0
+    prev["th"] += 300.0
+    now["qv"] = np.abs(now["qv"]) * 1e-3
+    _write_nc(tmp_path / "a.nc", prev, 0.5)
+    _write_nc(tmp_path / "b.nc", now, -3.0)
+    st = V.init_from_previous(cfg, SOUNDING, str(tmp_path / "a.nc"), str(tmp_path / "b.nc"))
+    for f in ("zeta", "qv", "qc", "qr"):
+        np.testing.assert_array_equal(st[f], now[f], err_msg=f)
+        np.testing.assert_array_equal(st[f + "m"], prev[f], err_msg=f + "m")
+    np.testing.assert_array_equal(st["u"], now["u"])
+    np.testing.assert_array_equal(st["w"], now["w"])
+    assert st.s["ubarTopm"] == 0.5
+    bubble = np.zeros((nx, nz))
+    if case_ == 1:
+        i = np.arange(nx)[:, None]
+        k = np.arange(nz)[None, :]
+        rad = np.sqrt(((i - 0.5) * 200.0 - nx * 200.0 / 2) ** 2 / 4000.0 ** 2 +
+                      ((k - 0.5) * 200.0 - 2500.0) ** 2 / 2000.0 ** 2)
+        bubble = np.where(rad <= 1, 1.5 * (np.cos(np.pi * rad) + 1), 0.0)
+    th = now["th"].copy()
+    th[1:-1, 1:-1] += bubble[1:-1, 1:-1]
+    th[0, 1:-1], th[-1, 1:-1] = th[-2, 1:-1], th[1, 1:-1]
+    th[:, 0], th[:, -1] = th[:, 1], th[:, -2]
+    np.testing.assert_allclose(st["th"], th, rtol=1e-15, atol=0)
+    thb = th[1:-1, 1:-1].mean(axis=0)
+    np.testing.assert_allclose(st["thb"][1:-1], thb, rtol=1e-14)
+    np.testing.assert_allclose(st["qvb"][1:-1], now["qv"][1:-1, 1:-1].mean(axis=0), rtol=1e-12)
+    sounding = np.loadtxt(SOUNDING, skiprows=2)[:nzi]
+    np.testing.assert_array_equal(st["rhou"][1:-1], sounding[:, 9])
+    np.testing.assert_array_equal(st["Q1LS"][1:-1], sounding[:, 7] * 6.0)
+
+
 def test_case2_shear_profile():
     """Closed form of Init2d's CASE 2 wind: -8 m/s at the surface, linear to
     0 at 2 km, constant above (src/Init.cpp:213-223)."""
