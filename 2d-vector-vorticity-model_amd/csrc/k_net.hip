@@ -101,7 +101,7 @@ __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict
         const long p = (long(b) * g.h + y) * g.w + x;
         if (MODE == 0) {
             const float z = acc[r] + bias[co];
-            outZ[p * NET_C + co] = to_bf16(z);
+            if (outZ) outZ[p * NET_C + co] = to_bf16(z);  // kept for training only
             outA[p * NET_C + co] = to_bf16(gelu(z));
         } else if (MODE == 1) {
             if (co == 0) outF[p] = acc[r] + bias[0];

@@ -129,7 +129,7 @@ int GuessNet::infer(const NetGeo &g, const float *feat, float *out, NetScratch &
         if (l == NET_L - 1)
             net_conv_last(g, buf[l & 1], wl, bl, out, s);
         else
-            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], sc.z, s);
+            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], nullptr, s);  // no pre-activations
     }
     return VVM_OK;
 }

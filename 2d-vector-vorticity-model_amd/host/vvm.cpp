@@ -454,9 +454,32 @@ void vvm::Output::output_nc(int n, vvm &model) {
     }
 }
 
+// src/Output.cpp:288-318: the five phase timers as double [x] variables
+// advection, poisson, diffusion, microphysics, all.  The reference sizes
+// the dimension nx while the arrays hold TIMEROUTPUTSIZE values; here the
+// dimension is the arrays' length, so nothing is read past them.
+void vvm::Output::output_time_nc(int n, vvm &model) {
+    const int T = model.TIMEROUTPUTSIZE > 0 ? model.TIMEROUTPUTSIZE : 1;
+    const std::vector<std::pair<std::string, int>> dims = {{"x", T}};
+    const std::vector<int> x = {0};
+    const std::vector<NcVar> vars = {
+        {"advection", x, model.t_advection, size_t(T)},
+        {"poisson", x, model.t_poisson, size_t(T)},
+        {"diffusion", x, model.t_diffusion, size_t(T)},
+        {"microphysics", x, model.t_microphysics, size_t(T)},
+        {"all", x, model.t_all, size_t(T)},
+    };
+    const std::string path = model.outputpath + "/timer/" + std::to_string(n) + ".nc";
+    if (!write_nc_classic(path, dims, vars)) {
+        std::fprintf(stderr, "vvm: cannot write %s\n", path.c_str());
+        std::exit(1);
+    }
+}
+
 void vvm::Output::create_all_directory(vvm &model) {
     if (model.outputpath.empty()) return;
-    std::string dirs[] = {model.outputpath, model.outputpath + "/state", model.outputpath + "/nc"};
+    std::string dirs[] = {model.outputpath, model.outputpath + "/state", model.outputpath + "/nc",
+                          model.outputpath + "/timer"};
     for (const auto &d : dirs)
         if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) {
             std::fprintf(stderr, "vvm: cannot create %s\n", d.c_str());

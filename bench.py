@@ -7,12 +7,14 @@ over the config-2 grid (2048 x 256 interior, dx = dz = 100 m, dt = 1 s,
 Gaussian theta noise, 100 W/m^2 surface heat flux, qv surface ramp), state
 resident in HBM.  The solve stops at ||b - A x|| / ||b|| <= 1e-10.
 
-Two paths are timed:
-  fp64_plain  -- FP64 MG-PCG from a zero guess (the reference's guess);
-  mixed       -- FP32 MG-PCG inside FP64 refinement from the best guess
-                 available (learned if trained weights exist, else the
-                 linear extrapolation of the last two solutions).
-`value` is the mixed path's throughput; both are printed.
+Three paths are timed:
+  fp64_plain     -- FP64 MG-PCG from a zero guess (the reference's guess);
+  mixed          -- FP32 MG-PCG inside FP64 refinement from the linear
+                    extrapolation of the last two solutions;
+  mixed_learned  -- the same solver from the conv-net guess, whose weights
+                    come from --net or are trained here first (untimed) on
+                    samples the model records from 1024x128 runs.
+`value` is the mixed_learned path's throughput; all three are printed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints one JSON line on stdout; progress goes to stderr.
@@ -96,6 +98,42 @@ def run_path(case: V.Case, state: V.HostState, steps: int, warmup: int, kernel: 
         final = state.copy()
         dev.download(final)
     return wall, stats, kavg, kn, final
+
+
+def guess_network(path: str | None, train_steps: int):
+    """The learned guess for the mixed_learned path: loaded from a file, or
+    trained here on the GPU (untimed) on samples the model records from its
+    own runs (1024x128, two seeds of 150 steps), evaluated on a held-out seed."""
+    net = V.Net()
+    if path:
+        net.load(path)
+        return net, {"source": path}
+    net.init_params(1)
+    t0 = time.perf_counter()
+    for seed in (11, 12):
+        c = V.grid_case(1024, 128, seed=seed, stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400,
+                        solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP)
+        with V.Device(c.cfg) as dev:
+            dev.upload(V.init_state(c))
+            dev.collect(150)
+            dev.step(150)
+            net.add_samples(dev)
+    c = V.grid_case(1024, 128, seed=19, stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400,
+                    solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP)
+    held = V.Device(c.cfg)
+    held.upload(V.init_state(c))
+    held.collect(60)
+    held.step(60)
+    e0 = net.eval(held)
+    ts = net.train(steps=train_steps, batch=16, crop=(64, 64), lr=2e-3, lr_final=5e-5, seed=5, log_every=500)
+    e1 = net.eval(held)
+    held.close()
+    info = {"samples": net.pool_size(), "adam_steps": train_steps, "train_seconds": ts.seconds,
+            "loss_first": ts.loss_first, "loss_last": ts.loss_last,
+            "heldout_sq_error_vs_extrapolation": e1, "untrained": e0,
+            "setup_seconds": time.perf_counter() - t0}
+    log(f"[bench] guess network: {info}")
+    return net, info
 
 
 def cpu_baseline(case: V.Case, state: V.HostState) -> dict | None:
@@ -212,7 +250,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel", default="diffusion", choices=sorted(KERNEL_BYTES_PER_POINT))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="fp64_plain,mixed")
+    ap.add_argument("--paths", default="fp64_plain,mixed,mixed_learned")
+    ap.add_argument("--train-steps", type=int, default=6000,
+                    help="Adam steps of the in-process training of the guess network (untimed)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -224,8 +264,8 @@ def main():
 
     paths = {
         "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
-        "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED,
-                      guess_mode=V.VVM_GUESS_LEARNED if args.net else V.VVM_GUESS_EXTRAP),
+        "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+        "mixed_learned": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_LEARNED),
     }
     wanted = [p for p in args.paths.split(",") if p]
     if args.calibrate_copy:
@@ -234,13 +274,13 @@ def main():
     state = V.init_state(base)
     pts = (base.cfg.nx - 2) * (base.cfg.nz - 2)
     results, finals = {}, {}
+    learned_info = None
     for name in wanted:
         c = V.case("c2", stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, **paths[name])
         log(f"[bench] path {name}: {args.warmup} warmup + {args.steps} timed steps of 2048x256")
         net = None
-        if args.net and name == "mixed":
-            net = V.Net()
-            net.load(args.net)
+        if name == "mixed_learned":
+            net, learned_info = guess_network(args.net, args.train_steps)
         wall, stats, kavg, kn, final = run_path(c, state, args.steps, args.warmup, args.kernel, net)
         finals[name] = final
         results[name] = {
@@ -256,7 +296,7 @@ def main():
         log(f"[bench] {name}: {results[name]['value']:.4g} grid-point-steps/s, "
             f"{results[name]['ms_per_step']:.3f} ms/step, {results[name]['solve_iters_per_step']:.2f} iters/step")
 
-    head = "mixed" if "mixed" in results else wanted[0]
+    head = next((p for p in ("mixed_learned", "mixed") if p in results), wanted[0])
     R = results[head]
     kbytes = KERNEL_BYTES_PER_POINT[args.kernel] * pts
     achieved = kbytes / (R["kernel_avg_us"] * 1e-6) / 1e9 if R["kernel_avg_us"] > 0 else 0.0
@@ -274,11 +314,14 @@ def main():
                      "bytes_per_launch": kbytes, "avg_launch_us": R["kernel_avg_us"],
                      "traffic": measured_traffic(args.kernel), "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT)},
     }
-    if "fp64_plain" in results and "mixed" in results:
-        out["speedup_mixed_vs_fp64"] = results["mixed"]["value"] / results["fp64_plain"]["value"]
-        a, b = finals["mixed"], finals["fp64_plain"]
-        out["mixed_vs_fp64_rel_l2"] = {f: float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300))
-                                       for f in ("zeta", "th", "qv", "u", "w")}
+    for p in ("mixed", "mixed_learned"):
+        if "fp64_plain" in results and p in results:
+            out[f"speedup_{p}_vs_fp64"] = results[p]["value"] / results["fp64_plain"]["value"]
+            a, b = finals[p], finals["fp64_plain"]
+            out[f"{p}_vs_fp64_rel_l2"] = {f: float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300))
+                                          for f in ("zeta", "th", "qv", "u", "w")}
+    if "mixed_learned" in results:
+        out["learned_guess"] = learned_info
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (one reference step at 2048x256, single thread)")
         cb = cpu_baseline(base, state)

@@ -224,6 +224,10 @@ class vvm {
         // rates as double [x][z], precip [x], and ubarTop (the reference
         // writes ubarTopp under that name).
         static void output_nc(int n, vvm &model);
+        // <outputpath>/timer/<n>.nc: the phase timers t_advection,
+        // t_poisson, t_diffusion, t_microphysics, t_all (src/Output.cpp:288-318).
+        static void output_time_nc(int n, vvm &model);
+        // <outputpath> with nc/, timer/ and state/ (src/Output.cpp:330-336).
         static void create_all_directory(vvm &model);
         // The model state at `step` as a VVMSTATE record file
         // <outputpath>/state/state_<step>.bin.

@@ -128,16 +128,22 @@ int main(int argc, char **argv) {
     for (int i = 0; i < m.nx; i++) m.precip[i] = 0.1 * i;
     m.ubarTopp = 1.25;
     m.ubarTop = -7.;
+    for (int t = 0; t < 10; t++) {
+        m.t_advection[t] = 1e-3 * t, m.t_poisson[t] = 2e-3 * t, m.t_diffusion[t] = 3e-3 * t;
+        m.t_microphysics[t] = 4e-3 * t, m.t_all[t] = 1e-2 * t;
+    }
     vvm::Output::create_all_directory(m);
     vvm::Output::output_nc(7, m);
+    vvm::Output::output_time_nc(7, m);
     return 0;
 }
 '''
 
 
 def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
-    """Output::output_nc writes the reference's per-step netCDF layout
-    (src/Output.cpp:49-104); a netCDF-3 reader (scipy) reads it back."""
+    """Output::output_nc and output_time_nc write the reference's per-step
+    netCDF layouts (src/Output.cpp:49-104, 288-318); a netCDF-3 reader
+    (scipy) reads them back."""
     built()
     src = tmp_path / "out.cpp"
     src.write_text(OUTPUT_PROG)
@@ -159,3 +165,10 @@ def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
         np.testing.assert_array_equal(nc.variables["condensation"].data, 3e-9 * (i - k))
         np.testing.assert_array_equal(nc.variables["precip"].data, 0.1 * np.arange(8))
         assert float(nc.variables["ubarTop"].data) == 1.25  # ubarTopp, as the reference writes it
+    # the timer file (src/Output.cpp:288-318), TIMEROUTPUTSIZE = 10 values per timer
+    with netcdf_file(str(tmp_path / "run" / "timer" / "7.nc"), "r", mmap=False) as nc:
+        assert nc.dimensions == {"x": 10}
+        assert list(nc.variables) == ["advection", "poisson", "diffusion", "microphysics", "all"]
+        t = np.arange(10)
+        np.testing.assert_array_equal(nc.variables["poisson"].data, 2e-3 * t)
+        np.testing.assert_array_equal(nc.variables["all"].data, 1e-2 * t)
