@@ -354,11 +354,14 @@ namespace {
 
 // netCDF classic (CDF-1) writer: big-endian header of dimensions and
 // variables without attributes, then each variable's doubles at its offset.
+// A dimension of length 0 is the record (unlimited) dimension: variables
+// whose first dimension it is are stored after the fixed ones, one slab
+// per record, records interleaved.
 struct NcVar {
     std::string name;
     std::vector<int> dims;  // dimension ids
     const double *data;
-    size_t count;
+    size_t count;           // values per record for a record variable
 };
 
 void put_be32(std::vector<unsigned char> &b, uint32_t v) {
@@ -371,13 +374,30 @@ void put_name(std::vector<unsigned char> &b, const std::string &s) {
     while (b.size() % 4) b.push_back(0);
 }
 
+void put_be64_doubles(std::vector<unsigned char> &buf, const double *p, size_t n) {
+    buf.resize(n * 8);
+    for (size_t i = 0; i < n; i++) {
+        uint64_t u;
+        std::memcpy(&u, &p[i], 8);
+        for (int b = 0; b < 8; b++) buf[i * 8 + b] = static_cast<unsigned char>(u >> (56 - 8 * b));
+    }
+}
+
+bool is_record_var(const std::vector<std::pair<std::string, int>> &dims, const NcVar &v) {
+    return !v.dims.empty() && dims[size_t(v.dims[0])].second == 0;
+}
+
+// A new file holding the fixed variables and, when there are record
+// variables, their first record.
 bool write_nc_classic(const std::string &path, const std::vector<std::pair<std::string, int>> &dims,
                       const std::vector<NcVar> &vars) {
     const uint32_t NC_DIMENSION = 10, NC_VARIABLE = 11, NC_DOUBLE = 6;
+    bool has_rec = false;
+    for (auto &v : vars) has_rec = has_rec || is_record_var(dims, v);
     // header size first (offsets depend on it)
     auto header = [&](const std::vector<uint32_t> &begins) {
         std::vector<unsigned char> h = {'C', 'D', 'F', 1};
-        put_be32(h, 0);  // numrecs
+        put_be32(h, has_rec ? 1 : 0);  // numrecs
         put_be32(h, NC_DIMENSION);
         put_be32(h, uint32_t(dims.size()));
         for (auto &d : dims) {
@@ -399,26 +419,157 @@ bool write_nc_classic(const std::string &path, const std::vector<std::pair<std::
         return h;
     };
     const size_t hsize = header({}).size();
-    std::vector<uint32_t> begins;
+    std::vector<uint32_t> begins(vars.size());
     size_t off = hsize;
-    for (auto &v : vars) {
-        if (off > 0xffffffffull) return false;  // CDF-1 offsets are 32-bit
-        begins.push_back(uint32_t(off));
-        off += v.count * 8;
-    }
+    for (int pass = 0; pass < 2; pass++)  // fixed variables, then the record
+        for (size_t v = 0; v < vars.size(); v++) {
+            if (is_record_var(dims, vars[v]) != (pass == 1)) continue;
+            if (off > 0xffffffffull) return false;  // CDF-1 offsets are 32-bit
+            begins[v] = uint32_t(off);
+            off += vars[v].count * 8;
+        }
     std::vector<unsigned char> h = header(begins);
     FILE *f = std::fopen(path.c_str(), "wb");
     if (!f) return false;
     bool ok = std::fwrite(h.data(), 1, h.size(), f) == h.size();
     std::vector<unsigned char> buf;
-    for (auto &v : vars) {
-        buf.resize(v.count * 8);
-        for (size_t i = 0; i < v.count; i++) {
-            uint64_t u;
-            std::memcpy(&u, &v.data[i], 8);
-            for (int b = 0; b < 8; b++) buf[i * 8 + b] = static_cast<unsigned char>(u >> (56 - 8 * b));
+    for (int pass = 0; pass < 2; pass++)
+        for (auto &v : vars) {
+            if (is_record_var(dims, v) != (pass == 1)) continue;
+            put_be64_doubles(buf, v.data, v.count);
+            ok = ok && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
         }
-        ok = ok && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    ok = (std::fclose(f) == 0) && ok;
+    return ok;
+}
+
+// What the appender needs of a classic (CDF-1 / CDF-2) header written by
+// any netCDF library: the record count, and per variable its type, slab
+// size, offset and whether it is a record variable.
+struct NcHeaderVar {
+    std::string name;
+    bool record = false;
+    uint32_t type = 0;
+    uint64_t vsize = 0, begin = 0;
+};
+
+struct NcHeader {
+    int version = 0;
+    uint32_t numrecs = 0;
+    std::vector<NcHeaderVar> vars;
+};
+
+bool read_nc_header(FILE *f, NcHeader &h) {
+    auto u32 = [&](uint32_t &v) {
+        unsigned char b[4];
+        if (std::fread(b, 1, 4, f) != 4) return false;
+        v = (uint32_t(b[0]) << 24) | (uint32_t(b[1]) << 16) | (uint32_t(b[2]) << 8) | b[3];
+        return true;
+    };
+    auto u64 = [&](uint64_t &v) {
+        uint32_t hi, lo;
+        if (!u32(hi) || !u32(lo)) return false;
+        v = (uint64_t(hi) << 32) | lo;
+        return true;
+    };
+    // n bytes of data padded to a multiple of 4
+    auto skip = [&](uint64_t n) { return std::fseek(f, long((n + 3) & ~uint64_t(3)), SEEK_CUR) == 0; };
+    auto name = [&](std::string &s) {
+        uint32_t n;
+        if (!u32(n) || n > 4096) return false;
+        s.assign(n, '\0');
+        if (n && std::fread(&s[0], 1, n, f) != n) return false;
+        return std::fseek(f, long((4 - n % 4) % 4), SEEK_CUR) == 0;
+    };
+    static const int type_size[7] = {0, 1, 1, 2, 4, 4, 8};
+    auto atts = [&]() {
+        uint32_t tag, n;
+        if (!u32(tag) || !u32(n)) return false;
+        if (tag == 0 && n == 0) return true;
+        if (tag != 12) return false;  // NC_ATTRIBUTE
+        for (uint32_t a = 0; a < n; a++) {
+            std::string s;
+            uint32_t type, nelems;
+            if (!name(s) || !u32(type) || !u32(nelems) || type < 1 || type > 6) return false;
+            if (!skip(uint64_t(nelems) * type_size[type])) return false;
+        }
+        return true;
+    };
+    unsigned char magic[4];
+    if (std::fread(magic, 1, 4, f) != 4 || magic[0] != 'C' || magic[1] != 'D' || magic[2] != 'F') return false;
+    h.version = magic[3];
+    if (h.version != 1 && h.version != 2) return false;
+    uint32_t tag, n;
+    if (!u32(h.numrecs) || !u32(tag) || !u32(n)) return false;
+    std::vector<uint32_t> dimlen;
+    if (tag == 10) {
+        for (uint32_t d = 0; d < n; d++) {
+            std::string s;
+            uint32_t len;
+            if (!name(s) || !u32(len)) return false;
+            dimlen.push_back(len);
+        }
+    } else if (tag != 0 || n != 0) {
+        return false;
+    }
+    if (!atts()) return false;
+    if (!u32(tag) || !u32(n)) return false;
+    if (tag != 11 && !(tag == 0 && n == 0)) return false;
+    for (uint32_t v = 0; tag == 11 && v < n; v++) {
+        NcHeaderVar var;
+        uint32_t nd;
+        if (!name(var.name) || !u32(nd) || nd > 64) return false;
+        for (uint32_t d = 0; d < nd; d++) {
+            uint32_t id;
+            if (!u32(id) || id >= dimlen.size()) return false;
+            if (d == 0 && dimlen[id] == 0) var.record = true;
+        }
+        uint32_t vs;
+        if (!atts() || !u32(var.type) || !u32(vs)) return false;
+        var.vsize = vs;
+        if (h.version == 2) {
+            if (!u64(var.begin)) return false;
+        } else {
+            uint32_t b;
+            if (!u32(b)) return false;
+            var.begin = b;
+        }
+        h.vars.push_back(var);
+    }
+    return true;
+}
+
+// One more record of every record variable in `vars` (all of them must be
+// record variables of type double with the file's slab size), then the
+// record count.
+bool append_nc_record(const std::string &path, const std::vector<NcVar> &vars) {
+    FILE *f = std::fopen(path.c_str(), "r+b");
+    if (!f) return false;
+    NcHeader h;
+    bool ok = read_nc_header(f, h);
+    uint64_t recsize = 0;
+    int nrec = 0;
+    for (auto &v : h.vars)
+        if (v.record) recsize += v.vsize, nrec++;
+    if (nrec == 1)  // a lone record variable is not padded
+        for (auto &v : h.vars)
+            if (v.record) recsize = v.vsize;
+    std::vector<unsigned char> buf;
+    for (auto &w : vars) {
+        if (!ok) break;
+        const NcHeaderVar *hv = nullptr;
+        for (auto &v : h.vars)
+            if (v.name == w.name) hv = &v;
+        ok = hv && hv->record && hv->type == 6 && hv->vsize == w.count * 8;
+        if (!ok) break;
+        put_be64_doubles(buf, w.data, w.count);
+        const uint64_t at = hv->begin + uint64_t(h.numrecs) * recsize;
+        ok = std::fseek(f, long(at), SEEK_SET) == 0 && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+    }
+    if (ok) {
+        std::vector<unsigned char> nr;
+        put_be32(nr, h.numrecs + 1);
+        ok = std::fseek(f, 4, SEEK_SET) == 0 && std::fwrite(nr.data(), 1, 4, f) == 4;
     }
     ok = (std::fclose(f) == 0) && ok;
     return ok;
@@ -426,8 +577,50 @@ bool write_nc_classic(const std::string &path, const std::vector<std::pair<std::
 
 }  // namespace
 
+namespace {
+
+// src/Output.cpp:106-285 (OUTPUTNCSAMEFILE) with Kessler: file number
+// n / 1200000, record n * dt; a file that exists gets one more record.
+void write_same_file_record(int n, vvm &model) {
+    const size_t f2 = size_t(model.nx) * model.nz;
+    const std::vector<std::pair<std::string, int>> dims = {{"t", 0}, {"x", model.nx}, {"z", model.nz}};
+    const std::vector<int> txz = {0, 1, 2}, tx = {0, 1}, t = {0};
+    const double tval = n * model.dt;
+    std::vector<NcVar> vars = {
+        {"t", t, &tval, 1},
+        {"th", txz, model.thcont, f2},
+        {"zeta", txz, model.zetacont, f2},
+        {"u", txz, model.ucont, f2},
+        {"w", txz, model.wcont, f2},
+        {"ubarTop", t, &model.ubarTopp, 1},
+        {"qv", txz, model.qvcont, f2},
+        {"qc", txz, model.qccont, f2},
+        {"qr", txz, model.qrcont, f2},
+        {"precip", tx, model.precip, size_t(model.nx)},
+    };
+    if (model.output_microphysics) {
+        vars.push_back({"accretion", txz, model.accretioncont, f2});
+        vars.push_back({"autoconversion", txz, model.autoconversioncont, f2});
+        vars.push_back({"evaporation", txz, model.evaporationcont, f2});
+        vars.push_back({"condensation", txz, model.condensationcont, f2});
+    }
+    const std::string path = model.outputpath + "/nc/" + std::to_string(n / 1200000) + ".nc";
+    struct stat sb;
+    const bool exists = stat(path.c_str(), &sb) == 0;
+    if (!(exists ? append_nc_record(path, vars) : write_nc_classic(path, dims, vars))) {
+        std::fprintf(stderr, "vvm: cannot %s %s\n", exists ? "append a record to" : "write", path.c_str());
+        std::exit(1);
+    }
+}
+
+}  // namespace
+
 void vvm::Output::output_nc(int n, vvm &model) {
     model.sync_to_host();
+    if (model.output_nc_same_file) {
+        write_same_file_record(n, model);
+        return;
+    }
     const size_t f2 = size_t(model.nx) * model.nz;
     const std::vector<std::pair<std::string, int>> dims = {{"x", model.nx}, {"z", model.nz}};
     const std::vector<int> xz = {0, 1}, x = {0};

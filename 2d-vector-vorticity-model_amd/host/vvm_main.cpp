@@ -23,7 +23,10 @@
 //   VVM_TOL_REL VVM_MAX_ITER VVM_GPU VVM_PRINT_STEPS (0|1)
 //   VVM_FINAL_STATE (path of a VVMSTATE file written after the run)
 //   VVM_OUTPUT_NC (1: <outputpath>/nc/<step>.nc on output steps, as the
-//     reference) VVM_OUTPUT_STATE (1: also <outputpath>/state/state_<step>.bin)
+//     reference) VVM_OUTPUT_NC_SAMEFILE (1: the OUTPUTNCSAMEFILE layout, one
+//     file with a record per output step) VVM_OUTPUT_MICROPHYSICS (1: the
+//     Kessler rates in that file) VVM_OUTPUT_STATE (1: also
+//     <outputpath>/state/state_<step>.bin)
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -141,6 +144,8 @@ int main(int argc, char **argv) {
     vvm model(config);
     model.print_steps = K.i("VVM_PRINT_STEPS", 1) != 0;
     model.output_nc_files = K.i("VVM_OUTPUT_NC", 1) != 0;
+    model.output_nc_same_file = K.i("VVM_OUTPUT_NC_SAMEFILE", 0) != 0;
+    model.output_microphysics = K.i("VVM_OUTPUT_MICROPHYSICS", 0) != 0;
     model.output_state_files = K.i("VVM_OUTPUT_STATE", 0) != 0;
     std::printf("nx=%d nz=%d dx=%g dz=%g dt=%g steps=%d solver=%s\n", model.nx, model.nz, model.dx, model.dz,
                 model.dt, int(model.TIMEEND / model.dt), config.solver_mode == VVM_SOLVER_MIXED ? "mixed" : "fp64");

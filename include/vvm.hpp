@@ -146,6 +146,8 @@ class vvm {
     std::string restart_prev_nc, restart_now_nc;  // LOADPATH1 / LOADPATH2 of the reference
     int start_step = 0;                // TIMENOW of the reference (TimeMarching's first step)
     bool output_nc_files = true;       // TimeMarching writes nc/<step>.nc on output steps
+    bool output_nc_same_file = false;  // OUTPUTNCSAMEFILE: append [t][x][z] slices to nc/<step/1200000>.nc
+    bool output_microphysics = false;  // OUTPUTMICROPHYSICS: Kessler rates in the same-file output
     bool output_state_files = false;   // ... and state/state_<step>.bin (exact restart)
     long long solve_iters_total = 0;   // PCG iterations over the run
     vvm_host_view host_view();          // pointers into this model's arrays
@@ -222,7 +224,12 @@ class vvm {
         // (src/Output.cpp:49-104): netCDF classic, dimensions x = nx and
         // z = nz, th, zeta, u, w, qv, qc, qr and the four Kessler process
         // rates as double [x][z], precip [x], and ubarTop (the reference
-        // writes ubarTopp under that name).
+        // writes ubarTopp under that name).  With output_nc_same_file it
+        // is the OUTPUTNCSAMEFILE variant (src/Output.cpp:106-285): one
+        // file nc/<n/1200000>.nc with an unlimited dimension t, a record
+        // t = n dt, th, zeta, u, w, qv, qc, qr as [t][x][z], ubarTop [t],
+        // precip [t][x] (and the Kessler rates with output_microphysics),
+        // created on first use and appended to afterwards.
         static void output_nc(int n, vvm &model);
         // <outputpath>/timer/<n>.nc: the phase timers t_advection,
         // t_poisson, t_diffusion, t_microphysics, t_all (src/Output.cpp:288-318).

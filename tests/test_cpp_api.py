@@ -135,15 +135,23 @@ int main(int argc, char **argv) {
     vvm::Output::create_all_directory(m);
     vvm::Output::output_nc(7, m);
     vvm::Output::output_time_nc(7, m);
+    // the same-file layout: three records, the second and third appended
+    m.output_nc_same_file = true;
+    for (int r = 0; r < 3; r++) {
+        m.th[2][3] = 400. + r;
+        m.precip[5] = -1. * r;
+        m.ubarTopp = 10. + r;
+        vvm::Output::output_nc(600 * r, m);
+    }
     return 0;
 }
 '''
 
 
 def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
-    """Output::output_nc and output_time_nc write the reference's per-step
-    netCDF layouts (src/Output.cpp:49-104, 288-318); a netCDF-3 reader
-    (scipy) reads them back."""
+    """Output::output_nc and output_time_nc write the reference's netCDF
+    layouts (per step, src/Output.cpp:49-104; same file, :106-285; timers,
+    :288-318); a netCDF-3 reader (scipy) reads them back."""
     built()
     src = tmp_path / "out.cpp"
     src.write_text(OUTPUT_PROG)
@@ -155,6 +163,7 @@ def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
     r = subprocess.run([str(exe), str(tmp_path / "run")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     from scipy.io import netcdf_file
+    i, k = np.meshgrid(np.arange(8), np.arange(6), indexing="ij")
     with netcdf_file(str(tmp_path / "run" / "nc" / "7.nc"), "r", mmap=False) as nc:
         assert nc.dimensions == {"x": 8, "z": 6}
         assert list(nc.variables) == ["th", "zeta", "u", "w", "ubarTop", "qv", "qc", "qr", "accretion",
@@ -165,6 +174,18 @@ def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
         np.testing.assert_array_equal(nc.variables["condensation"].data, 3e-9 * (i - k))
         np.testing.assert_array_equal(nc.variables["precip"].data, 0.1 * np.arange(8))
         assert float(nc.variables["ubarTop"].data) == 1.25  # ubarTopp, as the reference writes it
+    # the same-file output (src/Output.cpp:106-285): t unlimited, records appended
+    with netcdf_file(str(tmp_path / "run" / "nc" / "0.nc"), "r", mmap=False) as nc:
+        assert nc.dimensions == {"t": None, "x": 8, "z": 6}
+        assert list(nc.variables) == ["t", "th", "zeta", "u", "w", "ubarTop", "qv", "qc", "qr", "precip"]
+        np.testing.assert_array_equal(nc.variables["t"].data, [0.0, 1200.0, 2400.0])  # n * dt, dt = 2
+        th = nc.variables["th"].data
+        assert th.shape == (3, 8, 6)
+        np.testing.assert_array_equal(th[:, 2, 3], [400.0, 401.0, 402.0])
+        np.testing.assert_array_equal(th[1, 4], 304.0 + 0.01 * np.arange(6))
+        np.testing.assert_array_equal(nc.variables["precip"].data[:, 5], [0.0, -1.0, -2.0])
+        np.testing.assert_array_equal(nc.variables["ubarTop"].data, [10.0, 11.0, 12.0])
+        np.testing.assert_array_equal(nc.variables["qr"].data[2], 1e-6 * (i + k))
     # the timer file (src/Output.cpp:288-318), TIMEROUTPUTSIZE = 10 values per timer
     with netcdf_file(str(tmp_path / "run" / "timer" / "7.nc"), "r", mmap=False) as nc:
         assert nc.dimensions == {"x": 10}
@@ -172,3 +193,39 @@ def test_output_nc_readable_by_a_netcdf_reader(tmp_path):
         t = np.arange(10)
         np.testing.assert_array_equal(nc.variables["poisson"].data, 2e-3 * t)
         np.testing.assert_array_equal(nc.variables["all"].data, 1e-2 * t)
+
+
+def test_same_file_output_appends_to_a_file_another_library_wrote(tmp_path):
+    """The same-file output opens an existing file and appends records, as
+    the reference's nc_open branch does (src/Output.cpp:137-168); here the
+    file was started by another writer (scipy) with two records."""
+    built()
+    from scipy.io import netcdf_file
+    (tmp_path / "run" / "nc").mkdir(parents=True)
+    names = ["t", "th", "zeta", "u", "w", "ubarTop", "qv", "qc", "qr", "precip"]
+    with netcdf_file(str(tmp_path / "run" / "nc" / "0.nc"), "w") as nc:
+        nc.createDimension("t", None)
+        nc.createDimension("x", 8)
+        nc.createDimension("z", 6)
+        for n in names:
+            dims = ("t",) if n in ("t", "ubarTop") else ("t", "x") if n == "precip" else ("t", "x", "z")
+            v = nc.createVariable(n, "d", dims)
+            v[:] = np.full((2,) + v.shape[1:], -5.0 if n != "t" else 0.0)
+    src = tmp_path / "out.cpp"
+    src.write_text(OUTPUT_PROG)
+    exe = tmp_path / "out"
+    root = os.path.dirname(PKG)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root}/include", str(src), "-o", str(exe), f"-L{PKG}",
+                        "-lvvm", f"-Wl,-rpath,{PKG}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe), str(tmp_path / "run")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    with netcdf_file(str(tmp_path / "run" / "nc" / "0.nc"), "r", mmap=False) as nc:
+        assert list(nc.variables) == names
+        np.testing.assert_array_equal(nc.variables["t"].data, [0.0, 0.0, 0.0, 1200.0, 2400.0])
+        th = nc.variables["th"].data
+        assert th.shape == (5, 8, 6)
+        assert (th[:2] == -5.0).all()
+        np.testing.assert_array_equal(th[2:, 2, 3], [400.0, 401.0, 402.0])
+        np.testing.assert_array_equal(nc.variables["ubarTop"].data, [-5.0, -5.0, 10.0, 11.0, 12.0])
+        np.testing.assert_array_equal(nc.variables["precip"].data[:, 5], [-5.0, -5.0, 0.0, -1.0, -2.0])
