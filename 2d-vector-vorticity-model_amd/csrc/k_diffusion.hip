@@ -393,59 +393,71 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
 #pragma unroll
     for (int q = 0; q < PFS; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
     bool gave_up = false;
-    for (int k0 = 1; k0 <= nz - 2; k0 += PFS) {
+    // one row from prefetch slot q; with `refill` the slot is then loaded
+    // again for row k + PFS
+    auto row = [&](const int k, const int q, const bool refill) {
+        double a[E], e[E];
+        Affine m{1., 0.};
 #pragma unroll
-        for (int q = 0; q < PFS; q++) {
-            const int k = k0 + q;
-            if (k > nz - 2) break;
-            double a[E], e[E];
-            Affine m{1., 0.};
-#pragma unroll
-            for (int j = 0; j < E; j++) {
-                a[j] = ba[q][j];
-                e[j] = bc[q][j] + bd[q][j] * ynew[j];
-                if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
-            }
-            load_row<E>(G, A_, C_, D_, i0, n, k + PFS, nz, ba[q], bc[q], bd[q]);
-            const Affine inc = wave_scan(m);
-            // exclusive prefix: the inclusive one of the lane below (identity at lane 0)
-            const double ea = dppd<0x138, 0xF>(1., inc.a), eb = dppd<0x138, 0xF>(0., inc.b);
-            double y0;
-            if (!src) {
-                y0 = ghost[k];  // ghost column, old as in the reference
-            } else {
-                unsigned long long bits = kEmpty;
-                if (lane == 0) {
-                    int spins = 0;
-                    while ((bits = lds_load(src + k)) == kEmpty && !gave_up) {
-                        if (++spins > kSpinLimit) {
-                            S.s[S_DIFF_ERR] = 1.;
-                            gave_up = true;
-                        }
-                        __builtin_amdgcn_s_sleep(1);
-                    }
-                }
-                // lane 0 is the first active lane: a scalar broadcast
-                const int lo = __builtin_amdgcn_readfirstlane(int(bits & 0xffffffffull));
-                const int hi = __builtin_amdgcn_readfirstlane(int(bits >> 32));
-                y0 = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-            }
-            double y = (lane == 0) ? y0 : ea * y0 + eb;
-            const double r = rate_l[k];
-#pragma unroll
-            for (int j = 0; j < E; j++) {
-                const int i = i0 + j;
-                if (i > n) continue;
-                y = a[j] * y + e[j];
-                ynew[j] = y;
-                double o = y;
-                if (r != 0.) o = is_theta ? y - r * (y - thbi[k]) : y - r * y;
-                f[ix(G, i, k)] = o;
-                if (i == strip1 && publish)
-                    lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
-            }
+        for (int j = 0; j < E; j++) {
+            a[j] = ba[q][j];
+            e[j] = bc[q][j] + bd[q][j] * ynew[j];
+            if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
         }
+        const Affine inc = wave_scan(m);
+        // exclusive prefix: the inclusive one of the lane below (identity at lane 0)
+        const double ea = dppd<0x138, 0xF>(1., inc.a), eb = dppd<0x138, 0xF>(0., inc.b);
+        double y0;
+        if (!src) {
+            y0 = ghost[k];  // ghost column, old as in the reference
+        } else {
+            unsigned long long bits = kEmpty;
+            if (lane == 0) {
+                int spins = 0;
+                while ((bits = lds_load(src + k)) == kEmpty && !gave_up) {
+                    if (++spins > kSpinLimit) {
+                        S.s[S_DIFF_ERR] = 1.;
+                        gave_up = true;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            // lane 0 is the first active lane: a scalar broadcast
+            const int lo = __builtin_amdgcn_readfirstlane(int(bits & 0xffffffffull));
+            const int hi = __builtin_amdgcn_readfirstlane(int(bits >> 32));
+            y0 = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+        }
+        double y = (lane == 0) ? y0 : ea * y0 + eb;
+        const double r = rate_l[k];
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int i = i0 + j;
+            if (i > n) continue;
+            y = a[j] * y + e[j];
+            ynew[j] = y;
+            double o = y;
+            if (r != 0.) o = is_theta ? y - r * (y - thbi[k]) : y - r * y;
+            f[ix(G, i, k)] = o;
+            if (i == strip1 && publish)
+                lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
+        }
+        // refill after the last use of the slot, so that the loaded values
+        // stay in the slot's own registers (no copy that would wait for them)
+        if (refill) load_row<E>(G, A_, C_, D_, i0, n, k + PFS, nz, ba[q], bc[q], bd[q]);
+    };
+    // Whole groups of PFS rows first, with no exit inside a group: an exit
+    // between the rows of an unrolled group makes the compiler copy each
+    // refilled slot at once, which waits for its load and undoes the
+    // prefetch.  The last, partial group refills nothing.
+    const int kend = nz - 2;
+    int k0 = 1;
+    for (; k0 + PFS - 1 <= kend; k0 += PFS) {
+#pragma unroll
+        for (int q = 0; q < PFS; q++) row(k0 + q, q, true);
     }
+#pragma unroll
+    for (int q = 0; q < PFS; q++)
+        if (k0 + q <= kend) row(k0 + q, q, false);
 }
 
 

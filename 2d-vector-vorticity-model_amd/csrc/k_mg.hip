@@ -45,11 +45,19 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
                                                  const T *__restrict__ dinv, T omega, int Nc, int Mc,
                                                  T *__restrict__ bc) {
     __shared__ T Bs[DH * DW], X1[DH * DW], X2[DH * DW];
+    __shared__ T Rx[DH], Ru[DH], Rd[DH];  // cx, cu, dinv of the tile's rows
     const int ic0 = blockIdx.x * CX, jc0 = 1 + blockIdx.y * CY;
     const int ic1 = min(ic0 + CX - 1, Nc - 1), jc1 = min(jc0 + CY - 1, Mc);
     const int fi0 = 2 * ic0 - 3, fj0 = 2 * jc0 - 3;
     const int W = 2 * (ic1 - ic0) + 7, H = 2 * (jc1 - jc0) + 7;
     const int n = W * H;
+    if (threadIdx.x < H) {  // loaded with the tile, so later steps read LDS only
+        const int j = fj0 + threadIdx.x;
+        const bool in = (j >= 1 && j <= M);
+        Rx[threadIdx.x] = in ? cx[j] : T(0);
+        Ru[threadIdx.x] = (j >= 0 && j <= M) ? cu[j] : T(0);
+        Rd[threadIdx.x] = in ? dinv[j] : T(0);
+    }
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         const int tj = t / W, ti = t % W;
         const int j = fj0 + tj, i = wrapi(fi0 + ti, N);
@@ -65,7 +73,7 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
         T v = T(0);
         if (ti >= 1 && ti <= W - 2 && tj >= 1 && tj <= H - 2 && j >= 1 && j <= M) {
             const int s = tj * DW + ti;
-            v = X1[s] + omega * dinv[j] * (Bs[s] - lap(X1, DW, s, cx[j], cu[j], cu[j - 1]));
+            v = X1[s] + omega * Rd[tj] * (Bs[s] - lap(X1, DW, s, Rx[tj], Ru[tj], Ru[tj - 1]));
         }
         X2[tj * DW + ti] = v;
     }
@@ -85,7 +93,7 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
         const int j = fj0 + tj;
         if (ti >= 2 && ti <= W - 3 && tj >= 2 && tj <= H - 3) {
             const int s = tj * DW + ti;
-            X1[s] = (j >= 1 && j <= M) ? Bs[s] - lap(X2, DW, s, cx[j], cu[j], cu[j - 1]) : T(0);
+            X1[s] = (j >= 1 && j <= M) ? Bs[s] - lap(X2, DW, s, Rx[tj], Ru[tj], Ru[tj - 1]) : T(0);
         }
     }
     __syncthreads();
@@ -117,9 +125,17 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
                                                const T *__restrict__ cu, const T *__restrict__ dinv, T omega, int Nc,
                                                const T *__restrict__ xc, RedSlot rs, int dot) {
     __shared__ T Xs[UH * UW], X3[UH * UW], Bs[UH * UW];
+    __shared__ T Rx[UH], Ru[UH], Rd[UH];  // cx, cu, dinv of the tile's rows
     const int i0 = blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
     const int fi0 = i0 - 2, fj0 = j0 - 2;
     const int n = UW * UH;
+    if (threadIdx.x < UH) {  // loaded with the tile, so later steps read LDS only
+        const int j = fj0 + threadIdx.x;
+        const bool in = (j >= 1 && j <= M);
+        Rx[threadIdx.x] = in ? cx[j] : T(0);
+        Ru[threadIdx.x] = (j >= 0 && j <= M) ? cu[j] : T(0);
+        Rd[threadIdx.x] = in ? dinv[j] : T(0);
+    }
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         const int tj = t / UW, ti = t % UW;
         const int j = fj0 + tj, i = wrapi(fi0 + ti, N);
@@ -140,7 +156,7 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
         const int j = fj0 + tj;
         T v = T(0);
         if (ti >= 1 && ti <= UW - 2 && tj >= 1 && tj <= UH - 2 && j >= 1 && j <= M)
-            v = Xs[t] + omega * dinv[j] * (Bs[t] - lap(Xs, UW, t, cx[j], cu[j], cu[j - 1]));
+            v = Xs[t] + omega * Rd[tj] * (Bs[t] - lap(Xs, UW, t, Rx[tj], Ru[tj], Ru[tj - 1]));
         X3[t] = v;
     }
     __syncthreads();
@@ -148,8 +164,8 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
     for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
         const int i = i0 + t % TX, j = j0 + t / TX;
         if (i >= N || j > M) continue;
-        const int s = (j - fj0) * UW + (i - fi0);
-        const T z = X3[s] + omega * dinv[j] * (Bs[s] - lap(X3, UW, s, cx[j], cu[j], cu[j - 1]));
+        const int s = (j - fj0) * UW + (i - fi0), tj = j - fj0;
+        const T z = X3[s] + omega * Rd[tj] * (Bs[s] - lap(X3, UW, s, Rx[tj], Ru[tj], Ru[tj - 1]));
         xout[size_t(j) * N + i] = z;
         part[0] += double(Bs[s]) * double(z);
     }
@@ -168,6 +184,11 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T 
     const int L = cp.nlev;
     auto B = [&](int l) { return lds + cp.offB[l]; };
     auto X = [&](int l) { return lds + cp.offX[l]; };
+    // the level coefficients, staged here once: every step below then reads
+    // only LDS, with no global-memory latency between two barriers
+    auto CX = [&](int l) { return lds + cp.offC[l]; };
+    auto CU = [&](int l) { return lds + cp.offC[l] + (cp.M[l] + 2); };
+    auto DI = [&](int l) { return lds + cp.offC[l] + 2 * (cp.M[l] + 2); };
     T *Tm = lds + cp.offT;
     // b of the first level; zero every x (its Dirichlet rows stay zero)
     {
@@ -179,11 +200,21 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T 
         for (int l = 0; l < L; l++)
             for (int t = threadIdx.x; t < (cp.M[l] + 2) * cp.N[l]; t += blockDim.x) X(l)[t] = T(0);
         for (int t = threadIdx.x; t < cp.Tsize; t += blockDim.x) Tm[t] = T(0);
+        if (cp.offI >= 0)
+            for (int t = threadIdx.x; t < cp.cinv_n * cp.cinv_n; t += blockDim.x) lds[cp.offI + t] = cp.cinv[t];
+        for (int l = 0; l < L; l++) {
+            const int m2 = cp.M[l] + 2;
+            for (int t = threadIdx.x; t < m2; t += blockDim.x) {
+                CX(l)[t] = cp.cx[l][t];
+                if (t < m2 - 1) CU(l)[t] = cp.cu[l][t];
+                DI(l)[t] = cp.dinv[l][t];
+            }
+        }
     }
     __syncthreads();
     auto sweep = [&](int l, const T *xin, T *xo) {
         const int N = cp.N[l], M = cp.M[l];
-        const T *cx = cp.cx[l], *cu = cp.cu[l], *di = cp.dinv[l], *b = B(l);
+        const T *cx = CX(l), *cu = CU(l), *di = DI(l), *b = B(l);
         for (int t = threadIdx.x; t < M * N; t += blockDim.x) {
             const int j = 1 + t / N, i = t % N;
             const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
@@ -204,13 +235,13 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T 
         const int N = cp.N[l], M = cp.M[l];
         for (int t = threadIdx.x; t < M * N; t += blockDim.x) {
             const int j = 1 + t / N, s = j * N + t % N;
-            X(l)[s] = om * cp.dinv[l][j] * B(l)[s];
+            X(l)[s] = om * DI(l)[j] * B(l)[s];
         }
         __syncthreads();
         sweep(l, X(l), Tm);
         // residual into X(l) (x2 stays in Tm), then restriction
         {
-            const T *cx = cp.cx[l], *cu = cp.cu[l], *b = B(l);
+            const T *cx = CX(l), *cu = CU(l), *b = B(l);
             for (int t = threadIdx.x; t < M * N; t += blockDim.x) {
                 const int j = 1 + t / N, i = t % N;
                 const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
@@ -246,8 +277,9 @@ __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T 
         const int l = L - 1;
         const int N = cp.N[l], M = cp.M[l], n = N * M;
         if (cp.cinv && cp.cinv_n == n) {
+            const T *ci = (cp.offI >= 0) ? lds + cp.offI : cp.cinv;
             for (int t = threadIdx.x; t < n; t += blockDim.x) {
-                const T *row = cp.cinv + size_t(t) * n;
+                const T *row = ci + size_t(t) * n;
                 T s = T(0);
                 for (int q = 0; q < n; q++) s += row[q] * B(l)[N + q];  // rows 1..M start at offset N
                 X(l)[N + t] = s;
@@ -305,7 +337,7 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
         size_t tot = 0, tmax = 0;
         for (int l = lc; l < H.nlev; l++) {
             const size_t s = size_t(H.lev[l].M + 2) * H.lev[l].N;
-            tot += 2 * s;
+            tot += 2 * s + 3 * size_t(H.lev[l].M + 2);
             tmax = std::max(tmax, s);
         }
         if ((tot + tmax) * sizeof(T) > lds_budget || H.nlev - lc > kMaxCoarse) break;
@@ -333,12 +365,23 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
         off += s;
         tmax = std::max(tmax, s);
     }
+    for (int q = 0; q < cp.nlev; q++) {
+        cp.offC[q] = off;
+        off += 3 * (cp.M[q] + 2);
+    }
     cp.offT = off;
     cp.Tsize = tmax;
     cp.omega = H.omega;
     cp.cinv = H.cinv;
     cp.cinv_n = H.cinv_n;
     cp.coarse_iters = H.coarse_iters;
+    // a small dense inverse (up to 32 x 32) is staged in LDS with the rest
+    cp.offI = -1;
+    if (H.cinv && H.cinv_n <= 32 && !global &&
+        size_t(off + tmax + H.cinv_n * H.cinv_n) * sizeof(T) <= 64 * 1024) {
+        cp.offI = off + tmax;
+        tmax += H.cinv_n * H.cinv_n;
+    }
     H.coarse_lds = size_t(off + tmax) * sizeof(T);
     if (global) {
         if (hipMalloc(reinterpret_cast<void **>(&H.cws), H.coarse_lds) != hipSuccess) {

@@ -22,11 +22,13 @@ struct CoarseParams {
     int N[kMaxCoarse] = {}, M[kMaxCoarse] = {};
     const T *cx[kMaxCoarse] = {}, *cu[kMaxCoarse] = {}, *dinv[kMaxCoarse] = {};
     int offB[kMaxCoarse] = {}, offX[kMaxCoarse] = {};
+    int offC[kMaxCoarse] = {};  // cx [M+2], cu [M+1], dinv [M+2] of each level, 3 (M+2) apart
     int offT = 0, Tsize = 0;
     T omega = T(0.8);
     int coarse_iters = 32;
     const T *cinv = nullptr;  // dense inverse of the coarsest level (or null)
     int cinv_n = 0;
+    int offI = -1;            // its LDS copy when small (else -1: read from cinv)
 };
 
 template <typename T>
