@@ -113,23 +113,42 @@ __device__ inline double ab2_hist(double *const (&h)[2], size_t p, int step, dou
 // ------------------------------------------------------------------ zeta
 
 // Iteration::pzeta_pt: Advection_zeta (Advection.cpp:84-149) + Bouyancy
-// (Buoyancy.cpp:11-58) + BoundaryProcess2D_westdown(zetap).
-__global__ void __launch_bounds__(BX *BY) k_pzeta(Grid G, StepPtrs S, int step) {
-    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
-    if (i >= G.nx || k >= G.nz) return;
+// (Buoyancy.cpp:11-58) + BoundaryProcess2D_westdown(zetap).  As in
+// k_thermo4, each vorticity flux is formed once per face: every thread forms
+// the flux on its own east and upper faces, the first two waves the faces on
+// the tile's west and lower edges, and after a barrier each point takes the
+// differences.
+constexpr int PZX = 64, PZY = 8;
+
+__global__ void __launch_bounds__(PZX *PZY) k_pzeta(Grid G, StepPtrs S, int step) {
+    __shared__ double Fx[PZY][PZX + 1];  // slot c: x-flux east of column i0 + c - 1
+    __shared__ double Fz[PZY + 1][PZX];  // slot r: z-flux above row k0 + r - 1
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * PZX, k0 = blockIdx.y * PZY;
+    const int i = i0 + tx, k = k0 + ty;
+    const int nx = G.nx, nz = G.nz;
+    const double *z = S.now[F_ZETA], *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
+    // flux_u at i-1 = 0 is the periodic image of nx-2 (BoundaryProcess2D_center
+    // on flux_u): the ghost column 0 holds it for column 1
+    if (k >= 2 && k <= nz - 2 && i <= nx - 2) Fx[ty][tx + 1] = zflux_x(G, z, u, rhou, (i == 0) ? nx - 2 : i, k);
+    if (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2) Fz[ty + 1][tx] = zflux_z(G, z, w, rhow, i, k);
+    if (ty == 0 && tx < PZY) {  // west edge: the face east of column i0 - 1
+        const int kr = k0 + tx;
+        if (i0 >= 1 && i0 <= nx - 2 && kr >= 2 && kr <= nz - 2) Fx[tx][0] = zflux_x(G, z, u, rhou, i0 - 1, kr);
+    } else if (ty == 1) {  // lower edge: the face above row k0 - 1
+        if (k0 >= 2 && k0 <= nz - 2 && i >= 1 && i <= nx - 2) Fz[0][tx] = zflux_z(G, z, w, rhow, i, k0 - 1);
+    }
+    __syncthreads();
+    if (i >= nx || k >= nz) return;
     double *zp = S.fut[F_ZETA];
-    if (k == 0 || k == 1 || k == G.nz - 1) {
+    if (k == 0 || k == 1 || k == nz - 1) {
         zp[ix(G, i, k)] = 0.;
         return;
     }
-    if (i < 1 || i > G.nx - 2) return;
-    const double *z = S.now[F_ZETA], *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
+    if (i < 1 || i > nx - 2) return;
     const size_t p = ix(G, i, k);
-    // flux_u at i-1 = 0 is the periodic image of nx-2 (BoundaryProcess2D_center on flux_u)
-    const double fu0 = zflux_x(G, z, u, rhou, i, k);
-    const double fum = zflux_x(G, z, u, rhou, (i == 1) ? G.nx - 2 : i - 1, k);
-    const double fw0 = zflux_z(G, z, w, rhow, i, k);
-    const double fwm = zflux_z(G, z, w, rhow, i, k - 1);
+    const double fu0 = Fx[ty][tx + 1], fum = Fx[ty][tx];
+    const double fw0 = Fz[ty + 1][tx], fwm = Fz[ty][tx];
     const double px = (fu0 - fum) * G.r2dx / rhow[k];
     const double pz = (fw0 - fwm) * G.r2dz / rhow[k];
     const double dadv = -px - pz;
@@ -181,40 +200,85 @@ __device__ inline double vt_flux(const Grid &G, const double *qr, const double *
 // precipitation, and the first pass of NegativeValueProcess (zero the
 // negatives, sum positives and negatives) for qv, qc, qr.  k_finalize turns
 // the sums into rescale ratios.
-__global__ void __launch_bounds__(BX *BY) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
-    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
-    const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
+//
+// Each face flux is formed once, as the reference forms its flux arrays:
+// a workgroup covers T4X x T4Y points; first every thread forms the x-face
+// flux on its west face and the z-face fluxes (four fields and the rain
+// fall) on its lower face, and a few waves form the faces on the tile's east
+// and top edges; after a barrier each point takes the differences.  (Forming
+// both faces of every point would evaluate each flux, with its square roots
+// and powers, twice.)
+constexpr int T4X = 64, T4Y = 8;
+
+__global__ void __launch_bounds__(T4X *T4Y) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
+    __shared__ double Fx[4][T4Y][T4X + 1];  // x-face flux at faces i0 .. i0+T4X (west faces)
+    __shared__ double Fz[5][T4Y + 1][T4X];  // z-face flux (4 fields, rain fall) at faces k0 .. k0+T4Y
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = blockIdx.x * T4X, k0 = blockIdx.y * T4Y;
+    const int i = i0 + tx, k = k0 + ty;
+    const int nx = G.nx, nz = G.nz;
+    const double *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
+    const double *fl[4] = {S.now[F_TH], S.now[F_QV], S.now[F_QC], S.now[F_QR]};
+    const bool row_in = (k >= 1 && k <= nz - 2), inside = row_in && i >= 1 && i <= nx - 2;
+    // the ghost column nx-1 holds the face east of column nx-2: the
+    // periodic image of face 1
+    if (row_in && i >= 1 && i <= nx - 1) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) Fx[q][ty][tx] = sflux_x(G, fl[q], u, rhou, (i == nx - 1) ? 1 : i, k);
+    }
+    if (inside) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) Fz[q][ty][tx] = sflux_z(G, fl[q], w, rhow, i, k);
+        Fz[4][ty][tx] = vt_flux(G, fl[3], rhow, S.P.vtfac, i, k);
+    }
+    // edge faces, one kind per wave: waves 0-3 the top faces of field ty,
+    // wave 4 the top rain-fall faces, wave 5 the east faces (row, field)
+    {
+        const int kt = k0 + T4Y, ie = i0 + T4X;
+        if (ty < 5) {
+            if (i >= 1 && i <= nx - 2 && kt <= nz - 2)
+                Fz[ty][T4Y][tx] = (ty < 4) ? sflux_z(G, fl[ty], w, rhow, i, kt) : vt_flux(G, fl[3], rhow, S.P.vtfac, i, kt);
+        } else if (ty == 5 && tx < 4 * T4Y) {
+            const int r = tx % T4Y, q = tx / T4Y, kr = k0 + r;
+            // the face east of column nx-2 is the periodic image of face 1
+            if (ie - 1 <= nx - 2 && kr >= 1 && kr <= nz - 2)
+                Fx[q][r][T4X] = sflux_x(G, fl[q], u, rhou, (ie == nx - 1) ? 1 : ie, kr);
+        }
+    }
+    __syncthreads();
     double part[6] = {0., 0., 0., 0., 0., 0.};
     if (inside) {
-        const double *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
         const size_t p = ix(G, i, k);
-        // theta
-        {
-            const double d = thermo_tend(G, S.now[F_TH], u, w, rhou, rhow, i, k);
-            const double dold = ab2_hist(S.hist[H_TH_ADV], p, step, d);
-            S.fut[F_TH][p] = S.now[F_TH][p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
-        }
-        const int hid[3] = {H_QV_ADV, H_QC_ADV, H_QR_ADV};
+        const int hid[4] = {H_TH_ADV, H_QV_ADV, H_QC_ADV, H_QR_ADV};
+        const bool top = (k + 1 == nz - 1);
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int f = F_QV + q;
-            const double d = thermo_tend(G, S.now[f], u, w, rhou, rhow, i, k);
+        for (int q = 0; q < 4; q++) {
+            // Advection_thermo's tendency (Advection.cpp:55-81)
+            const double fu1 = Fx[q][ty][tx + 1], fu0 = Fx[q][ty][tx];
+            const double fw1 = top ? 0. : Fz[q][ty + 1][tx];
+            const double fw0 = Fz[q][ty][tx];
+            const double px = (fu1 - fu0) * G.r2dx / rhou[k];
+            const double pz = (fw1 - fw0) * G.r2dz / rhou[k];
+            const double d = -px - pz;
             const double dold = ab2_hist(S.hist[hid[q]], p, step, d);
-            double v = S.now[f][p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
-            if (f == F_QR) {
-                const double *qr = S.now[F_QR];
-                const double f1 = (k + 1 == G.nz - 1) ? 0. : vt_flux(G, qr, rhow, S.P.vtfac, i, k + 1);
-                const double f0 = vt_flux(G, qr, rhow, S.P.vtfac, i, k);
+            double v = fl[q][p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
+            if (q == 0) {
+                S.fut[F_TH][p] = v;
+                continue;
+            }
+            if (q == 3) {
+                const double f1 = top ? 0. : Fz[4][ty + 1][tx];
+                const double f0 = Fz[4][ty][tx];
                 const double dvt = (f1 - f0) * G.r2dz / rhou[k];
                 const double vold = ab2_hist(S.hist[H_QR_VT], p, step, dvt);
                 v += 1.5 * G.dt * dvt - 0.5 * G.dt * vold;
             }
-            if (v >= 0.) part[2 * q] += v;
+            if (v >= 0.) part[2 * (q - 1)] += v;
             else {
-                part[2 * q + 1] += v;
+                part[2 * (q - 1) + 1] += v;
                 v = 0.;
             }
-            S.fut[f][p] = v;
+            S.fut[F_QV + q - 1][p] = v;
         }
         if (k == 1) {
             // surface precipitation, src/Iteration.cpp:114-119 (now-qr, last w)
@@ -326,11 +390,13 @@ inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY
 }  // namespace
 
 void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
-    hipLaunchKernelGGL(k_pzeta, grid2d(G), dim3(BX, BY), 0, st, G, S, step);
+    const dim3 g((G.nx + PZX - 1) / PZX, (G.nz + PZY - 1) / PZY);
+    hipLaunchKernelGGL(k_pzeta, g, dim3(PZX, PZY), 0, st, G, S, step);
 }
 void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st) {
-    hipLaunchKernelGGL(k_thermo4, grid2d(G), dim3(BX, BY), 0, st, G, S, step, rs);
-    finalize<6>(rs, nblocks(grid2d(G)), S.s, RED_NEGVAL, S_NEG_RATIO, st);
+    const dim3 g((G.nx + T4X - 1) / T4X, (G.nz + T4Y - 1) / T4Y);
+    hipLaunchKernelGGL(k_thermo4, g, dim3(T4X, T4Y), 0, st, G, S, step, rs);
+    finalize<6>(rs, nblocks(g), S.s, RED_NEGVAL, S_NEG_RATIO, st);
 }
 void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
                           hipStream_t st) {
