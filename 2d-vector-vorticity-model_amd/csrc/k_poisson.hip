@@ -294,11 +294,12 @@ __global__ void __launch_bounds__(1024) k_calu_top(Grid G, StepPtrs S) {
 
 // Downward integration u[i][k] = u[i][nz-2] - dz sum_{k'=k}^{nz-3}
 // ((w[i][k'+1]-w[i-1][k'+1]) rdx - rhow[k'+1] zetap[i][k'+1]) (:643-650):
-// 64 columns x 16 vertical chunks per block, chunk sums scanned in LDS.
-constexpr int UC = 16;
-__global__ void __launch_bounds__(64 * UC) k_calu_cols(Grid G, StepPtrs S) {
-    __shared__ double cs[UC][64];
-    const int i = 1 + blockIdx.x * 64 + threadIdx.x, c = threadIdx.y;
+// UW columns x UC vertical chunks per block (narrow blocks, so that the
+// 2-D grid has enough workgroups for every CU), chunk sums scanned in LDS.
+constexpr int UW = 16, UC = 64;
+__global__ void __launch_bounds__(UW *UC) k_calu_cols(Grid G, StepPtrs S) {
+    __shared__ double cs[UC][UW];
+    const int i = 1 + blockIdx.x * UW + threadIdx.x, c = threadIdx.y;
     const int K = G.nz - 3;  // levels nz-3 .. 1
     const int C = (K + UC - 1) / UC;
     const bool col = (i <= G.nx - 2);
@@ -714,7 +715,7 @@ bool cal_u_supported(const Grid &G) { return G.nx - 2 <= 8192; }
 void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st) {
     const int N = G.nx - 2;
     hipLaunchKernelGGL(k_calu_top, dim3(1), dim3(1024), size_t(N) * sizeof(double), st, G, S);
-    hipLaunchKernelGGL(k_calu_cols, dim3((N + 63) / 64), dim3(64, UC), 0, st, G, S);
+    hipLaunchKernelGGL(k_calu_cols, dim3((N + UW - 1) / UW), dim3(UW, UC), 0, st, G, S);
 }
 
 }  // namespace vvmd
