@@ -125,7 +125,8 @@ enum RedMode {
 
 namespace {
 template <int NV>
-__global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *sc, int mode, int dst, int dst2) {
+__global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *sc, int mode, int dst, int dst2,
+                                                  double *hout) {
     double acc[NV];
 #pragma unroll
     for (int j = 0; j < NV; j++) acc[j] = 0.;
@@ -140,6 +141,9 @@ __global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *
             for (int j = 0; j < NV; j++) {
                 sc[dst + j] = acc[j];
                 if (dst2 >= 0) sc[dst2 + j] = acc[j];
+                // host-mapped coherent memory: the host reads it after an
+                // event, with no copy in the stream
+                if (hout) __hip_atomic_store(hout + j, acc[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             }
             break;
         case RED_NEGVAL:
@@ -166,8 +170,22 @@ __global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *
 }  // namespace
 
 template <int NV>
-inline void finalize(const RedSlot &rs, int nblk, double *sc, int mode, int dst, hipStream_t st, int dst2 = -1) {
-    hipLaunchKernelGGL(k_finalize<NV>, dim3(1), dim3(256), 0, st, rs, nblk, sc, mode, dst, dst2);
+inline void finalize(const RedSlot &rs, int nblk, double *sc, int mode, int dst, hipStream_t st, int dst2 = -1,
+                     double *hout = nullptr) {
+    hipLaunchKernelGGL(k_finalize<NV>, dim3(1), dim3(256), 0, st, rs, nblk, sc, mode, dst, dst2, hout);
+}
+
+// Sum of a grid's block partials (row 0 of rs, nblk of them) inside every
+// block of the consuming kernel, in a fixed order, so that all blocks agree
+// bit for bit; this saves a one-block k_finalize launch between producer
+// and consumer when nblk is small.
+__device__ inline double block_total(const RedSlot &rs, int nblk) {
+    const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+    const int nthr = blockDim.x * blockDim.y * blockDim.z;
+    double v[1] = {0.};
+    for (int b = tid; b < nblk; b += nthr) v[0] += rs.partial[b];
+    block_sum<1>(v);
+    return v[0];
 }
 
 inline int nblocks(dim3 g) { return int(g.x * g.y * g.z); }

@@ -35,42 +35,56 @@ __device__ inline T applyL(const T *__restrict__ x, int N, int i, int j, T cxj, 
 
 // ------------------------------------------------------------------ CG
 
-// p = z + beta p (beta = 0 on the first iteration), q = L p, alpha = rz/(p.q).
+// The two CG vector kernels run on at most kCGBlocks workgroups, each
+// walking LX x LY tiles, so that the update kernel can add the p.q partials
+// of every pq workgroup itself (no k_finalize between them).
+constexpr int kCGBlocks = 1024;
+inline int cg_blocks(int N, int M) { return std::min(kCGBlocks, int(nblocks(lgrid(N, M)))); }
+
+// p = z + beta p (beta = 0 on the first iteration), q = L p, partials of p.q
+// into rs_pq.
 template <typename T>
 __global__ void __launch_bounds__(LX *LY) k_pcg_pq(int N, int M, const T *z, const T *pin, T *pout, T *q, const T *cx,
-                                                   const T *cu, double *sc, RedSlot rs, int first) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+                                                   const T *cu, double *sc, RedSlot rs_pq, int first) {
+    const int tiles_x = (N + LX - 1) / LX, ntiles = tiles_x * ((M + LY - 1) / LY);
+    const T beta = first ? T(0) : T(sc[P_BETA]);
     double part[1] = {0.};
-    if (i < N && j <= M) {
-        const T beta = first ? T(0) : T(sc[P_BETA]);
-        const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
-        const size_t r = size_t(j) * N;
-        auto pn = [&](size_t a) -> T { return first ? z[a] : z[a] + beta * pin[a]; };
-        const T c = pn(r + i);
-        const T Lp = cx[j] * (T(2) * c - pn(r + im) - pn(r + ip)) + cu[j] * (c - pn(r + N + i)) +
-                     cu[j - 1] * (c - pn(r - N + i));
-        pout[r + i] = c;
-        q[r + i] = Lp;
-        part[0] = double(c) * double(Lp);
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int i = (t % tiles_x) * LX + threadIdx.x, j = 1 + (t / tiles_x) * LY + threadIdx.y;
+        if (i < N && j <= M) {
+            const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
+            const size_t r = size_t(j) * N;
+            auto pn = [&](size_t a) -> T { return first ? z[a] : z[a] + beta * pin[a]; };
+            const T c = pn(r + i);
+            const T Lp = cx[j] * (T(2) * c - pn(r + im) - pn(r + ip)) + cu[j] * (c - pn(r + N + i)) +
+                         cu[j - 1] * (c - pn(r - N + i));
+            pout[r + i] = c;
+            q[r + i] = Lp;
+            part[0] += double(c) * double(Lp);
+        }
     }
-    block_partials<1>(part, rs);
+    block_partials<1>(part, rs_pq);
 }
 
-// x += alpha p, r -= alpha q, first smoothing sweep z = omega D^-1 r, and
-// ||w r||^2 with the row weights w (rhow: the reference system's residual).
+// alpha = rz / p.q (p.q from the npq partials of k_pcg_pq), x += alpha p,
+// r -= alpha q, and ||w r||^2 with the row weights w (rhow: the reference
+// system's residual) as partials in rs.
 template <typename T>
-__global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r, const T *p, const T *q, T *z,
-                                                       const T *dinv, const T *wD, T omega, double *sc, RedSlot rs) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+__global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r, const T *p, const T *q,
+                                                       const T *wD, double *sc, RedSlot rs_pq, int npq, RedSlot rs) {
+    const int tiles_x = (N + LX - 1) / LX, ntiles = tiles_x * ((M + LY - 1) / LY);
+    const T alpha = T(sc[P_RZ] / block_total(rs_pq, npq));
     double part[1] = {0.};
-    if (i < N && j <= M) {
-        const size_t c = size_t(j) * N + i;
-        const T alpha = T(sc[P_ALPHA]);
-        x[c] = x[c] + alpha * p[c];
-        const T rn = r[c] - alpha * q[c];
-        r[c] = rn;
-        const double wr = double(wD[j]) * double(rn);
-        part[0] = wr * wr;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int i = (t % tiles_x) * LX + threadIdx.x, j = 1 + (t / tiles_x) * LY + threadIdx.y;
+        if (i < N && j <= M) {
+            const size_t c = size_t(j) * N + i;
+            x[c] = x[c] + alpha * p[c];
+            const T rn = r[c] - alpha * q[c];
+            r[c] = rn;
+            const double wr = double(wD[j]) * double(rn);
+            part[0] += wr * wr;
+        }
     }
     block_partials<1>(part, rs);
 }
@@ -459,8 +473,12 @@ void free_hier(MGHier<T> &H) {
 // iteration count; *rn2 gets the last residual norm^2 seen.
 template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
-             hipEvent_t ev, hipStream_t st, double *rn2, bool *conv) {
-    const dim3 g = lgrid(P.N, P.M), b(LX, LY);
+             double *hs_dev, hipEvent_t ev, hipStream_t st, double *rn2, bool *conv) {
+    const int nb = cg_blocks(P.N, P.M);
+    const dim3 b(LX, LY);
+    // p.q partials in the last row of the reduction slots, which the update
+    // kernel reads while it writes its own partials to row 0
+    const RedSlot rs_pq{rs.partial + size_t(7) * rs.max_blocks, rs.max_blocks};
     mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st);
     int it = 0;
     double best = 1e300;
@@ -468,18 +486,17 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
     *conv = false;
     while (it < max_iter) {
         it++;
-        hipLaunchKernelGGL(k_pcg_pq<T>, g, b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
-                           P.mg.lev[0].cu, P.sc, rs, int(it == 1));
-        finalize<1>(rs, nblocks(g), P.sc, RED_ALPHA, 0, st);
+        hipLaunchKernelGGL(k_pcg_pq<T>, dim3(nb), b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
+                           P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1));
         std::swap(P.pA, P.pB);
-        hipLaunchKernelGGL(k_pcg_update<T>, g, b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.zA, P.mg.lev[0].dinv,
-                           P.wD, P.mg.omega, P.sc, rs);
-        finalize<1>(rs, nblocks(g), P.sc, RED_STORE, P_RNORM2, st);
-        hipMemcpyAsync(hs, P.sc + P_RNORM2, sizeof(double), hipMemcpyDeviceToHost, st);
+        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.wD, P.sc, rs_pq,
+                           nb, rs);
+        // ||w r||^2 into sc and straight into the pinned host word hs
+        finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
         hipEventRecord(ev, st);
         if (it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
         hipEventSynchronize(ev);
-        const double rn = *hs;
+        const double rn = *static_cast<volatile double *>(hs);
         *rn2 = rn;
         if (!(rn > target)) {
             *conv = true;
@@ -552,8 +569,12 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
     xprev = dalloc<double>(n);
     xprev2 = dalloc<double>(n);
     if (!f || !xprev || !xprev2) return VVM_ERR_NOMEM;
-    if (hipHostMalloc(reinterpret_cast<void **>(&hscal), 16 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+    // pinned, coherent and mapped: k_finalize writes the CG residual norm
+    // here directly, and the host reads it after an event
+    if (hipHostMalloc(reinterpret_cast<void **>(&hscal), 16 * sizeof(double),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         return VVM_ERR_NOMEM;
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&hscal_dev), hscal, 0) != hipSuccess) return VVM_ERR_HIP;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return VVM_ERR_HIP;
     return VVM_OK;
 }
@@ -581,6 +602,7 @@ void Poisson::destroy() {
     net = nullptr;
     if (hscal) hipHostFree(hscal);
     hscal = nullptr;
+    hscal_dev = nullptr;
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
 }
@@ -660,7 +682,7 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     }
     double last = rn2;
     bool conv = false;
-    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, ev, st, &last, &conv);
+    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, ev, st, &last, &conv);
     R.rel = std::sqrt(last / bn2);
     R.converged = conv;
     return R;
@@ -702,7 +724,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         const double r0 = hscal[P_RNORM2];
         double last = r0;
         bool conv = false;
-        R.iters += pcg_loop(F, inner_tol * inner_tol * r0, max_inner, false, red, hscal + 8, ev, st, &last, &conv);
+        R.iters += pcg_loop(F, inner_tol * inner_tol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, ev, st, &last, &conv);
         hipLaunchKernelGGL(k_add_correction, g, b, 0, st, N, M, D.x, F.x);
     }
     return R;

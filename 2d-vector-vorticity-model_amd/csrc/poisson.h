@@ -99,6 +99,7 @@ class Poisson {
     double *xprev2 = nullptr;
     double *r64 = nullptr;     // FP64 residual of the refinement loop
     double *hscal = nullptr;   // pinned host copy of scalars
+    double *hscal_dev = nullptr;  // the same memory as the device sees it
     RedSlot red;
     float *gfeat = nullptr;    // [3][M][N] features of the learned guess
     float *gdelta = nullptr;   // [M][N] network output

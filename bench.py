@@ -64,12 +64,15 @@ def measured_traffic(kernel: str) -> float | None:
     table (None if the table lacks a kernel)."""
     if not os.path.exists(TRAFFIC_TABLE):
         return None
-    rows = {}
+    rows, col = {}, None
     for line in open(TRAFFIC_TABLE):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 5 and cells[0].startswith("`"):
+        if col is None:  # the header names the column of scaled HBM bytes
+            col = next((n for n, c in enumerate(cells) if c.startswith("HBM bytes")), None)
+            continue
+        if len(cells) > col and cells[0].startswith("`"):
             try:
-                rows[cells[0].strip("`").split("<")[0]] = float(cells[4])
+                rows[cells[0].strip("`").split("<")[0]] = float(cells[col])
             except ValueError:
                 pass
     names = KERNEL_NAMES.get(kernel, [])
