@@ -393,20 +393,81 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
 #pragma unroll
     for (int q = 0; q < PFS; q++) load_row<E>(G, A_, C_, D_, i0, n, 1 + q, nz, ba[q], bc[q], bd[q]);
     bool gave_up = false;
+    // The scan of a row's affine maps y -> a y + e splits in two chains: the
+    // products of the a's (P), which do not depend on the row below, and
+    // the offsets (B), which do.  P of row k+1 is formed while row k's B
+    // chain runs (independent instructions the compiler interleaves with
+    // the dependent ones), so the per-row critical path is the B chain
+    // alone: one DPP move pair, a multiply and an add per step.  Same
+    // operations, same rounding as the combined affine scan.
+    double Pc[7];  // P at the start of scan steps 1..6 and the inclusive P, current row
+    auto pchain = [&](const int q, double (&P)[7]) {
+        double p = 1.;
+#pragma unroll
+        for (int j = 0; j < E; j++)
+            if (i0 + j <= n) p = ba[q][j] * p;
+        P[0] = p;
+        p = p * dppd<0x111, 0xF>(1., p);
+        P[1] = p;
+        p = p * dppd<0x112, 0xF>(1., p);
+        P[2] = p;
+        p = p * dppd<0x114, 0xF>(1., p);
+        P[3] = p;
+        p = p * dppd<0x118, 0xF>(1., p);
+        P[4] = p;
+        p = p * dppd<0x142, 0xA>(1., p);
+        P[5] = p;
+        p = p * dppd<0x143, 0xC>(1., p);
+        P[6] = p;
+    };
+    pchain(0, Pc);
     // one row from prefetch slot q; with `refill` the slot is then loaded
     // again for row k + PFS
     auto row = [&](const int k, const int q, const bool refill) {
         double a[E], e[E];
-        Affine m{1., 0.};
+        double b = 0.;
 #pragma unroll
         for (int j = 0; j < E; j++) {
             a[j] = ba[q][j];
             e[j] = bc[q][j] + bd[q][j] * ynew[j];
-            if (i0 + j <= n) m = after(Affine{a[j], e[j]}, m);
+            if (i0 + j <= n) b = a[j] * b + e[j];
         }
-        const Affine inc = wave_scan(m);
+        // the B chain of this row, step by step beside the P chain of the
+        // next row (written interleaved so that the two fill each other's
+        // latency)
+        const int qn = (q + 1) % PFS;
+        double pn = 1.;
+#pragma unroll
+        for (int j = 0; j < E; j++)
+            if (i0 + j <= n) pn = ba[qn][j] * pn;
+        double Pn[7];
+        Pn[0] = pn;
+        const double d1 = dppd<0x111, 0xF>(0., b), p1 = dppd<0x111, 0xF>(1., pn);  // row_shr:1
+        b = Pc[0] * d1 + b;
+        pn = pn * p1;
+        Pn[1] = pn;
+        const double d2 = dppd<0x112, 0xF>(0., b), p2 = dppd<0x112, 0xF>(1., pn);  // row_shr:2
+        b = Pc[1] * d2 + b;
+        pn = pn * p2;
+        Pn[2] = pn;
+        const double d3 = dppd<0x114, 0xF>(0., b), p3 = dppd<0x114, 0xF>(1., pn);  // row_shr:4
+        b = Pc[2] * d3 + b;
+        pn = pn * p3;
+        Pn[3] = pn;
+        const double d4 = dppd<0x118, 0xF>(0., b), p4 = dppd<0x118, 0xF>(1., pn);  // row_shr:8
+        b = Pc[3] * d4 + b;
+        pn = pn * p4;
+        Pn[4] = pn;
+        const double d5 = dppd<0x142, 0xA>(0., b), p5 = dppd<0x142, 0xA>(1., pn);  // row_bcast:15, rows 1, 3
+        b = Pc[4] * d5 + b;
+        pn = pn * p5;
+        Pn[5] = pn;
+        const double d6 = dppd<0x143, 0xC>(0., b), p6 = dppd<0x143, 0xC>(1., pn);  // row_bcast:31, rows 2, 3
+        b = Pc[5] * d6 + b;
+        pn = pn * p6;
+        Pn[6] = pn;
         // exclusive prefix: the inclusive one of the lane below (identity at lane 0)
-        const double ea = dppd<0x138, 0xF>(1., inc.a), eb = dppd<0x138, 0xF>(0., inc.b);
+        const double ea = dppd<0x138, 0xF>(1., Pc[6]), eb = dppd<0x138, 0xF>(0., b);
         double y0;
         if (!src) {
             y0 = ghost[k];  // ghost column, old as in the reference
@@ -441,6 +502,8 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
             if (i == strip1 && publish)
                 lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
         }
+#pragma unroll
+        for (int t = 0; t < 7; t++) Pc[t] = Pn[t];
         // refill after the last use of the slot, so that the loaded values
         // stay in the slot's own registers (no copy that would wait for them)
         if (refill) load_row<E>(G, A_, C_, D_, i0, n, k + PFS, nz, ba[q], bc[q], bd[q]);

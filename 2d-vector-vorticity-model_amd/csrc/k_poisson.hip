@@ -468,12 +468,17 @@ void free_hier(MGHier<T> &H) {
 }
 
 // Preconditioned CG iterations on P (x, r and sc[P_RNORM2] set up by the
-// caller) until ||w r||^2 <= target or max_iter.  The convergence test lags
-// one V-cycle behind the GPU so the stream never drains.  Returns the
-// iteration count; *rn2 gets the last residual norm^2 seen.
+// caller; rn_start = that ||w r||^2) until ||w r||^2 <= target or max_iter.
+// The convergence test lags one V-cycle behind the GPU so the stream never
+// drains: the next V-cycle is queued before the host reads the residual.
+// When the residual history predicts that this iteration meets the target
+// (geometric extrapolation of the last ratio), the V-cycle is queued only
+// after the test, since a speculative V-cycle after the last iteration is
+// wasted work; a wrong prediction costs one drained queue instead.  Returns
+// the iteration count; *rn2 gets the last residual norm^2 seen.
 template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
-             double *hs_dev, hipEvent_t ev, hipStream_t st, double *rn2, bool *conv) {
+             double *hs_dev, hipEvent_t ev, hipStream_t st, double rn_start, double *rn2, bool *conv) {
     const int nb = cg_blocks(P.N, P.M);
     const dim3 b(LX, LY);
     // p.q partials in the last row of the reduction slots, which the update
@@ -483,6 +488,7 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
     int it = 0;
     double best = 1e300;
     int stall = 0;
+    double prev = rn_start, prev2 = -1.;
     *conv = false;
     while (it < max_iter) {
         it++;
@@ -494,7 +500,14 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
         // ||w r||^2 into sc and straight into the pinned host word hs
         finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
         hipEventRecord(ev, st);
-        if (it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
+        bool ahead = it < max_iter;
+        if (ahead && !stall_stop) {
+            // an MG-PCG step cuts ||r||^2 by ~1e-2; the last observed ratio is a
+            // better guess once there is one
+            const double q = (prev2 > 0.) ? std::min(prev / prev2, 1.) : 1e-2;
+            if (prev * q <= target) ahead = false;
+        }
+        if (ahead) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
         hipEventSynchronize(ev);
         const double rn = *static_cast<volatile double *>(hs);
         *rn2 = rn;
@@ -502,6 +515,9 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             *conv = true;
             break;
         }
+        if (!ahead && it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
+        prev2 = prev;
+        prev = rn;
         if (stall_stop) {
             if (rn < 0.25 * best) {
                 best = rn;
@@ -682,7 +698,7 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     }
     double last = rn2;
     bool conv = false;
-    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, ev, st, &last, &conv);
+    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, ev, st, rn2, &last, &conv);
     R.rel = std::sqrt(last / bn2);
     R.converged = conv;
     return R;
@@ -732,7 +748,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         const double r0 = rn2;
         double last = r0;
         bool conv = false;
-        R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, ev, st, &last,
+        R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, ev, st, r0, &last,
                             &conv);
         hipLaunchKernelGGL(k_add_correction, g, b, 0, st, N, M, D.x, F.x);
     }
