@@ -8,6 +8,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import subprocess
+import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
@@ -87,9 +88,12 @@ def ref_available() -> bool:
 
 
 def run_ref(mode: str, arg: str, in_path: str, out_path: str, timeout: float = 600) -> None:
-    """Run the compiled reference harness (oracle/_ref/vvm_ref), single thread."""
+    """Run the compiled reference harness (oracle/_ref/vvm_ref), single thread.
+    Its own prints go to stderr, so a caller's stdout carries only what the
+    caller writes."""
     env = dict(os.environ, OMP_NUM_THREADS="1")
-    subprocess.run([REF_BIN, mode, str(arg), in_path, out_path], check=True, env=env, timeout=timeout)
+    subprocess.run([REF_BIN, mode, str(arg), in_path, out_path], check=True, env=env, timeout=timeout,
+                   stdout=sys.stderr)
 
 
 # ------------------------------------------------------- guess network
