@@ -61,20 +61,6 @@ __global__ void k_guess_normalize(size_t plane, double npts, const double *__res
     dst[t] = float(double(feat[t]) * inv_rms(gsc[c], npts));
 }
 
-__global__ void __launch_bounds__(GX *GY) k_guess_apply(int N, int M, const double *__restrict__ x1,
-                                                        const double *__restrict__ x2, const float *__restrict__ delta,
-                                                        const double *__restrict__ gsc, double *__restrict__ x) {
-    const int i = blockIdx.x * GX + threadIdx.x, j = blockIdx.y * GY + threadIdx.y;
-    if (i >= N || j > M + 1) return;
-    const size_t a = size_t(j) * N + i;
-    if (j == 0 || j == M + 1) {
-        x[a] = 0.;
-        return;
-    }
-    const double sd = sqrt(gsc[1] / (double(N) * M));
-    x[a] = xe_at(x1, x2, a) + sd * double(delta[size_t(j - 1) * N + i]);
-}
-
 __global__ void __launch_bounds__(GX *GY) k_guess_target(int N, int M, const double *__restrict__ x,
                                                          const double *__restrict__ x1,
                                                          const double *__restrict__ x2,
@@ -93,7 +79,6 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
     const size_t plane = size_t(M) * N;
     if (!gfeat) {
         hipMalloc(reinterpret_cast<void **>(&gfeat), sizeof(float) * 3 * plane);
-        hipMalloc(reinterpret_cast<void **>(&gdelta), sizeof(float) * plane);
         hipMalloc(reinterpret_cast<void **>(&gsc), sizeof(double) * 4);
     }
     if (!dst) dst = gfeat;  // in place: each element reads and writes its own slot
@@ -107,12 +92,23 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
 
 bool Poisson::learned_guess(hipStream_t st) {
     if (!net || nsolves < 2) return false;
-    guess_features(nullptr, st);
-    const NetGeo g{1, M, N, 1};
-    if (net->infer(g, gfeat, gdelta, gscratch, st) != VVM_OK) return false;
-    hipLaunchKernelGGL(k_guess_apply, ggrid(N, M + 2), dim3(GX, GY), 0, st, N, M, xprev, xprev2, gdelta, gsc,
-                       pcg64.x);
-    return true;
+    const size_t plane = size_t(M) * N;
+    if (!gfeat) {
+        hipMalloc(reinterpret_cast<void **>(&gfeat), sizeof(float) * 3 * plane);
+        hipMalloc(reinterpret_cast<void **>(&gsc), sizeof(double) * 4);
+    }
+    // raw features and their sums of squares; the network's first layer
+    // scales them, its last writes the guess into x (rows 1..M; the
+    // Dirichlet rows of x stay zero)
+    const dim3 g = ggrid(N, M);
+    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
+                       pcg64.mg.lev[0].cu, gfeat, red);
+    finalize<3>(red, nblocks(g), gsc, RED_STORE, 0, st);
+    const double npts = double(plane);
+    const NetGeo geo{1, M, N, 1};
+    const NetFeatIn fin{gfeat, gsc, npts};
+    const NetApply ap{xprev, xprev2, gsc, pcg64.x, N, npts};
+    return net->infer_guess(geo, fin, ap, gscratch, st) == VVM_OK;
 }
 
 int Poisson::collect_reserve(int cap) {

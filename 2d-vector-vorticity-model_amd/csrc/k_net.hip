@@ -8,12 +8,13 @@
 // zero), so every layer is the same 32 -> 32 convolution.
 //
 // Forward / backward-data kernel (k_conv): a workgroup of 4 wavefronts
-// computes a 64 x 2 pixel tile; each wavefront owns 32 pixels of one row
-// and all 32 output channels, i.e. one v_mfma_f32_32x32x16_bf16 output
-// block, and walks K = 9 taps x 32 input channels in 18 steps of 16.  The
-// (64+2) x (2+2) input tile with its halo is staged once in LDS.  The
+// computes a 64 x 2, 4 or 8 pixel tile; each wavefront owns one to four
+// v_mfma_f32_32x32x16_bf16 output blocks of 32 pixels x 32 output channels,
+// and walks K = 9 taps x 32 input channels in 18 steps of 16.  The input
+// tile with its one-pixel halo is staged once in LDS.  The
 // B operand (weights) is pre-packed in the exact lane order of the
-// instruction, so a lane fetches its 8 weights as one 16-byte load.
+// instruction, so a lane fetches its 8 weights as one 16-byte load and
+// uses them for its four blocks.
 //
 // Weight-gradient kernel (k_conv_wgrad): one wavefront per tap; the GEMM
 // runs over pixels (K) with the input tile and the output gradient staged
@@ -29,15 +30,24 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int TW = 64, TH = 2;                 // output tile (columns x rows)
-constexpr int HW = TW + 2, HH = TH + 2;        // haloed tile
+constexpr int TW = 64, TH = 2;                 // weight-gradient tile (columns x rows)
+constexpr int HW = TW + 2, HH = TH + 2;        // its haloed tile
+constexpr int FTW = 64;                        // forward / backward-data tile width; its height
+constexpr int FHW = FTW + 2;                   // (2, 4 or 8 rows) is a template parameter
+__device__ inline double inv_rms(double s2, double n) { return s2 > 0. ? 1. / sqrt(s2 / n) : 0.; }
 constexpr int PXS = 40;                        // LDS pixel stride in bf16 (80 bytes)
 constexpr float GC = 0.7978845608028654f;      // sqrt(2/pi)
 constexpr float GA = 0.044715f;
 
-__device__ inline float gelu(float z) { return 0.5f * z * (1.f + tanhf(GC * (z + GA * z * z * z))); }
+// tanh(u) = 1 - 2 / (1 + e^{2u}) on the hardware exponential: a few
+// instructions instead of the library's full-accuracy tanhf, which made the
+// GELU epilogue the costliest part of a convolution.  Its error (~1e-7
+// absolute) is far below the bf16 rounding the activations get next; both
+// ends saturate correctly (e^{2u} -> inf gives 1, -> 0 gives -1).
+__device__ inline float tanh_fast(float u) { return 1.f - 2.f / (1.f + __expf(2.f * u)); }
+__device__ inline float gelu(float z) { return 0.5f * z * (1.f + tanh_fast(GC * (z + GA * z * z * z))); }
 __device__ inline float gelu_d(float z) {
-    const float t = tanhf(GC * (z + GA * z * z * z));
+    const float t = tanh_fast(GC * (z + GA * z * z * z));
     return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * GC * (1.f + 3.f * GA * z * z);
 }
 
@@ -59,55 +69,102 @@ __device__ inline long halo_src(const NetGeo &g, int b, int y0, int x0, int ry, 
 // MODE 0: hidden layer (bias, GELU; writes A and Z), 1: last layer (bias;
 // channel 0 to fp32), 2: backward data (times gelu'(Z of the layer below);
 // writes that layer's pre-activation gradient).
-template <int MODE>
+//
+// MODE 3: last layer writing the solver's guess (NetApply).  IN 1: the
+// input is the three raw FP32 feature planes (NetFeatIn), scaled and
+// rounded to bf16 while staged; the other 29 channels are zero.
+//
+// A workgroup of 4 wavefronts computes a 64 x FTH pixel tile from the
+// (66 x FTH+2)-pixel haloed input staged once in LDS; the tile holds 2 FTH
+// MFMA output blocks of 32 pixels x 32 channels, FTH / 2 per wavefront, and
+// a wavefront applies each weight fragment it loads to all of its blocks.
+// Taller tiles re-read less halo; shorter ones fit more workgroups per CU.
+template <int MODE, int IN = 0, int FTH = 4>
 __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict__ in,
                                               const bf16x8 *__restrict__ wpk, const float *__restrict__ bias,
                                               __bf16 *__restrict__ outA, __bf16 *__restrict__ outZ,
-                                              float *__restrict__ outF, const __bf16 *__restrict__ zbelow) {
-    __shared__ __attribute__((aligned(16))) __bf16 tile[HH * HW * PXS];
-    const int b = blockIdx.z, y0 = blockIdx.y * TH, x0 = blockIdx.x * TW;
-    // stage the haloed tile: 16-byte chunks, 4 per pixel
-    for (int c = threadIdx.x; c < HH * HW * 4; c += blockDim.x) {
-        const int px = c >> 2, part = c & 3;
-        const int ry = px / HW, rx = px % HW;
-        const long src = halo_src(g, b, y0, x0, ry, rx);
-        bf16x8 v = {};
-        if (src >= 0) v = *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8);
-        *reinterpret_cast<bf16x8 *>(tile + px * PXS + part * 8) = v;
+                                              float *__restrict__ outF, const __bf16 *__restrict__ zbelow,
+                                              NetFeatIn fin, NetApply ap) {
+    constexpr int FHH = FTH + 2, BPW = FTH / 2;  // haloed rows, output blocks per wavefront
+    __shared__ __attribute__((aligned(16))) __bf16 tile[FHH * FHW * PXS];
+    const int b = blockIdx.z, y0 = blockIdx.y * FTH, x0 = blockIdx.x * FTW;
+    if (IN == 0) {
+        // stage the haloed tile: 16-byte chunks, 4 per pixel
+        for (int c = threadIdx.x; c < FHH * FHW * 4; c += blockDim.x) {
+            const int px = c >> 2, part = c & 3;
+            const int ry = px / FHW, rx = px % FHW;
+            const long src = halo_src(g, b, y0, x0, ry, rx);
+            bf16x8 v = {};
+            if (src >= 0) v = *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8);
+            *reinterpret_cast<bf16x8 *>(tile + px * PXS + part * 8) = v;
+        }
+    } else {
+        const long hw = long(g.h) * g.w;
+        double sc[3];
+#pragma unroll
+        for (int c = 0; c < 3; c++) sc[c] = inv_rms(fin.sumsq[c], fin.npts);
+        for (int px = threadIdx.x; px < FHH * FHW; px += blockDim.x) {
+            const long src = halo_src(g, b, y0, x0, px / FHW, px % FHW);
+            bf16x8 lo = {}, z = {};
+            if (src >= 0) {
+                const long q = src - long(b) * hw;
+#pragma unroll
+                for (int c = 0; c < 3; c++)
+                    lo[c] = to_bf16(float(double(fin.feat[(long(b) * 3 + c) * hw + q]) * sc[c]));
+            }
+            bf16x8 *dst = reinterpret_cast<bf16x8 *>(tile + px * PXS);
+            dst[0] = lo;
+            dst[1] = z;
+            dst[2] = z;
+            dst[3] = z;
+        }
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int ty = wv >> 1, cb = (wv & 1) * 32;
     const int i = lane & 31, gk = lane >> 5;
-    f32x16 acc = {};
+    f32x16 acc[BPW] = {};  // block gq = BPW wv + q: row gq / 2, columns 32 (gq % 2) ..
 #pragma unroll
     for (int s = 0; s < 18; s++) {
         const int t = s >> 1, hh = s & 1;
         const int dy = t / 3 - 1, dx = t % 3 - 1;
-        const int px = (ty + 1 + dy) * HW + (cb + i + 1 + dx);
-        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
         const bf16x8 w = wpk[s * 64 + lane];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < BPW; q++) {
+            const int gq = BPW * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
+            const int px = (ty + 1 + dy) * FHW + (cb + i + 1 + dx);
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc[q], 0, 0, 0);
+        }
     }
     // D[m][n]: n = lane % 32 (output channel), m = 8 (r / 4) + 4 (lane / 32) + r % 4 (pixel)
     const int co = lane & 31;
-    const int y = y0 + ty;
-    if (y >= g.h) return;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int m = 8 * (r >> 2) + 4 * gk + (r & 3);
-        const int x = x0 + cb + m;
-        if (x >= g.w) continue;
-        const long p = (long(b) * g.h + y) * g.w + x;
-        if (MODE == 0) {
-            const float z = acc[r] + bias[co];
-            if (outZ) outZ[p * NET_C + co] = to_bf16(z);  // kept for training only
-            outA[p * NET_C + co] = to_bf16(gelu(z));
-        } else if (MODE == 1) {
-            if (co == 0) outF[p] = acc[r] + bias[0];
-        } else {
-            const float dz = acc[r] * gelu_d(from_bf16(zbelow[p * NET_C + co]));
-            outA[p * NET_C + co] = to_bf16(dz);
+    for (int q = 0; q < BPW; q++) {
+        const int gq = BPW * wv + q, y = y0 + (gq >> 1), cb = (gq & 1) * 32;
+        if (y >= g.h) continue;
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int m = 8 * (r >> 2) + 4 * gk + (r & 3);
+            const int x = x0 + cb + m;
+            if (x >= g.w) continue;
+            const long p = (long(b) * g.h + y) * g.w + x;
+            if (MODE == 0) {
+                const float z = acc[q][r] + bias[co];
+                if (outZ) outZ[p * NET_C + co] = to_bf16(z);  // kept for training only
+                outA[p * NET_C + co] = to_bf16(gelu(z));
+            } else if (MODE == 1) {
+                if (co == 0) outF[p] = acc[q][r] + bias[0];
+            } else if (MODE == 3) {
+                if (co == 0) {
+#pragma clang fp contract(off)
+                    const float o = acc[q][r] + bias[0];
+                    const size_t a = size_t(y + 1) * ap.N + x;
+                    ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+                }
+            } else {
+                const float dz = acc[q][r] * gelu_d(from_bf16(zbelow[p * NET_C + co]));
+                outA[p * NET_C + co] = to_bf16(dz);
+            }
         }
     }
 }
@@ -305,24 +362,54 @@ void net_pack_input(const NetGeo &g, const float *feat, __bf16 *x, hipStream_t s
     hipLaunchKernelGGL(k_pack_input, dim3(unsigned((n + 255) / 256)), dim3(256), 0, st, g, feat, x);
 }
 
-static dim3 conv_grid(const NetGeo &g) { return dim3((g.w + TW - 1) / TW, (g.h + TH - 1) / TH, g.nb); }
+// tile height of the forward / backward-data kernels: 2, 4 or 8 rows
+static int g_conv_rows = 2;
+void net_set_conv_rows(int rows) { g_conv_rows = rows; }
+int net_conv_rows() { return g_conv_rows; }
+
+template <int MODE, int IN, int FTH>
+static void conv_launch(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A,
+                        __bf16 *Z, float *F, const __bf16 *zb, const NetFeatIn &fin, const NetApply &ap,
+                        hipStream_t st) {
+    const dim3 grid((g.w + FTW - 1) / FTW, (g.h + FTH - 1) / FTH, g.nb);
+    hipLaunchKernelGGL((k_conv<MODE, IN, FTH>), grid, dim3(256), 0, st, g, in,
+                       reinterpret_cast<const bf16x8 *>(wpk), bias, A, Z, F, zb, fin, ap);
+}
+
+template <int MODE, int IN = 0>
+static void conv(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A, __bf16 *Z,
+                 float *F, const __bf16 *zb, const NetFeatIn &fin, const NetApply &ap, hipStream_t st) {
+    if (g_conv_rows == 2)
+        conv_launch<MODE, IN, 2>(g, in, wpk, bias, A, Z, F, zb, fin, ap, st);
+    else if (g_conv_rows == 8)
+        conv_launch<MODE, IN, 8>(g, in, wpk, bias, A, Z, F, zb, fin, ap, st);
+    else
+        conv_launch<MODE, IN, 4>(g, in, wpk, bias, A, Z, F, zb, fin, ap, st);
+}
 
 void net_conv_hidden(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A, __bf16 *Z,
                      hipStream_t st) {
-    hipLaunchKernelGGL(k_conv<0>, conv_grid(g), dim3(256), 0, st, g, in, reinterpret_cast<const bf16x8 *>(wpk), bias,
-                       A, Z, nullptr, nullptr);
+    conv<0>(g, in, wpk, bias, A, Z, nullptr, nullptr, NetFeatIn{}, NetApply{}, st);
 }
 
 void net_conv_last(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, float *out,
                    hipStream_t st) {
-    hipLaunchKernelGGL(k_conv<1>, conv_grid(g), dim3(256), 0, st, g, in, reinterpret_cast<const bf16x8 *>(wpk), bias,
-                       nullptr, nullptr, out, nullptr);
+    conv<1>(g, in, wpk, bias, nullptr, nullptr, out, nullptr, NetFeatIn{}, NetApply{}, st);
+}
+
+void net_conv_first_features(const NetGeo &g, const NetFeatIn &fin, const __bf16 *wpk, const float *bias, __bf16 *A,
+                             hipStream_t st) {
+    conv<0, 1>(g, nullptr, wpk, bias, A, nullptr, nullptr, nullptr, fin, NetApply{}, st);
+}
+
+void net_conv_last_apply(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, const NetApply &ap,
+                         hipStream_t st) {
+    conv<3>(g, in, wpk, bias, nullptr, nullptr, nullptr, nullptr, NetFeatIn{}, ap, st);
 }
 
 void net_conv_bwd(const NetGeo &g, const __bf16 *dz, const __bf16 *wpk_bwd, const __bf16 *zbelow, __bf16 *dzbelow,
                   hipStream_t st) {
-    hipLaunchKernelGGL(k_conv<2>, conv_grid(g), dim3(256), 0, st, g, dz, reinterpret_cast<const bf16x8 *>(wpk_bwd),
-                       nullptr, dzbelow, nullptr, nullptr, zbelow);
+    conv<2>(g, dz, wpk_bwd, nullptr, dzbelow, nullptr, nullptr, zbelow, NetFeatIn{}, NetApply{}, st);
 }
 
 void net_wgrad(const NetGeo &g, const __bf16 *x, const __bf16 *dz, float *partial, int nblk, float *grad, int layer,

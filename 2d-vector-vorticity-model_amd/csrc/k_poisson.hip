@@ -607,11 +607,11 @@ static void free_pcg(PCG<T> &P) {
 void Poisson::destroy() {
     free_pcg(pcg64);
     if (have32) free_pcg(pcg32);
-    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64, (void *)gfeat, (void *)gdelta,
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64, (void *)gfeat,
                     (void *)gsc, (void *)samples})
         if (p) hipFree(p);
     f = xprev = xprev2 = r64 = nullptr;
-    gfeat = gdelta = samples = nullptr;
+    gfeat = samples = nullptr;
     gsc = nullptr;
     gscratch.release();
     sample_cap = sample_count = 0;

@@ -39,7 +39,30 @@ struct CropDesc {
 };
 void net_gather(int nb, int ch, int cw, int margin, const CropDesc *d, float *feat, float *tgt, float *mask,
                 hipStream_t st);
+// Inference ends of the network fused with the guess: the first layer
+// reads the three raw FP32 feature planes and scales feature c by
+// 1/RMS = 1/sqrt(sumsq[c]/npts); the last layer writes the guess
+// x = (2 x1 - x2) + sqrt(sumsq[1]/npts) * out straight into the solver's
+// unknown (row y of the network is row y+1 of x, N columns).
+struct NetFeatIn {
+    const float *feat;
+    const double *sumsq;
+    double npts;
+};
+struct NetApply {
+    const double *x1, *x2, *sumsq;
+    double *x;
+    int N;
+    double npts;
+};
+void net_conv_first_features(const NetGeo &g, const NetFeatIn &fin, const __bf16 *wpk, const float *bias, __bf16 *A,
+                             hipStream_t st);
+void net_conv_last_apply(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, const NetApply &ap,
+                         hipStream_t st);
 void net_pack(const float *params, __bf16 *fwd, __bf16 *bwd, hipStream_t st);
+// tile height (2, 4 or 8 rows) of the convolution kernels, process-wide
+void net_set_conv_rows(int rows);
+int net_conv_rows();
 void net_pack_input(const NetGeo &g, const float *feat, __bf16 *x, hipStream_t st);
 void net_conv_hidden(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A, __bf16 *Z,
                      hipStream_t st);
@@ -68,6 +91,9 @@ class GuessNet {
    public:
     // inference with the caller's buffers: reads only the parameters
     int infer(const NetGeo &g, const float *feat, float *out, NetScratch &sc, hipStream_t s) const;
+    // the solver's guess in one pass: raw features in, guess written to ap.x
+    int infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply &ap, NetScratch &sc,
+                    hipStream_t s) const;
     int init(int device, hipStream_t st);
     void destroy();
     // activation buffers for up to npx pixels (grown on demand)

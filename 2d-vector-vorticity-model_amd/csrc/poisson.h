@@ -102,7 +102,6 @@ class Poisson {
     double *hscal_dev = nullptr;  // the same memory as the device sees it
     RedSlot red;
     float *gfeat = nullptr;    // [3][M][N] features of the learned guess
-    float *gdelta = nullptr;   // [M][N] network output
     double *gsc = nullptr;     // sums of squares of the three features
     NetScratch gscratch;       // this solver's network activations
     void guess_features(float *dst, hipStream_t st);

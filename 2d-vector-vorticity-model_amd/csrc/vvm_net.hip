@@ -134,6 +134,25 @@ int GuessNet::infer(const NetGeo &g, const float *feat, float *out, NetScratch &
     return VVM_OK;
 }
 
+int GuessNet::infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply &ap, NetScratch &sc,
+                          hipStream_t s) const {
+    const long npx = long(g.nb) * g.h * g.w;
+    int rc = sc.reserve(npx);
+    if (rc) return rc;
+    __bf16 *buf[2] = {sc.a0, sc.a1};
+    for (int l = 0; l < NET_L; l++) {
+        const __bf16 *wl = wf + size_t(l) * NET_PACK;
+        const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
+        if (l == 0)
+            net_conv_first_features(g, fin, wl, bl, buf[1], s);
+        else if (l == NET_L - 1)
+            net_conv_last_apply(g, buf[l & 1], wl, bl, ap, s);
+        else
+            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], nullptr, s);
+    }
+    return VVM_OK;
+}
+
 int GuessNet::forward(const NetGeo &g, const float *feat, float *out, bool keep, hipStream_t s) {
     const long npx = long(g.nb) * g.h * g.w;
     int rc = reserve(npx, keep);
@@ -359,6 +378,12 @@ int vvm_net_adam(vvm_net_t *n, float lr) {
     hipSetDevice(n->device);
     n->net.adam_step(lr);
     return hipStreamSynchronize(n->st) == hipSuccess ? VVM_OK : nfail(n, VVM_ERR_HIP, "sync failed");
+}
+
+int vvm_net_set_conv_rows(int rows) {
+    if (rows != 2 && rows != 4 && rows != 8) return VVM_ERR_ARG;
+    vvmd::net_set_conv_rows(rows);
+    return VVM_OK;
 }
 
 int vvm_net_add_samples(vvm_net_t *n, vvm_dev_t *h) {

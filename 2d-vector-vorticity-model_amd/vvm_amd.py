@@ -143,6 +143,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_net_grad_host": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, FPTR, FPTR, FPTR, FPTR,
                                         P(C.c_double)]),
         "vvm_net_adam": (C.c_int, [C.c_void_p, C.c_float]),
+        "vvm_net_set_conv_rows": (C.c_int, [C.c_int]),
         "vvm_dev_collect": (C.c_int, [C.c_void_p, C.c_int]),
         "vvm_dev_collected": (C.c_int, [C.c_void_p]),
         "vvm_net_add_samples": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -598,6 +599,13 @@ class Net:
 
     def adam(self, lr: float) -> None:
         self._check(self.lib.vvm_net_adam(self.h, C.c_float(lr)), "adam")
+
+    @staticmethod
+    def set_conv_rows(rows: int) -> None:
+        """Tile height (2, 4 or 8 rows) of every network's convolutions."""
+        rc = load_lib().vvm_net_set_conv_rows(int(rows))
+        if rc != 0:
+            raise ValueError(f"vvm_net_set_conv_rows({rows}) returned {rc}")
 
     def add_samples(self, dev: "Device") -> None:
         self._check(self.lib.vvm_net_add_samples(self.h, dev.h), "add_samples")

@@ -54,6 +54,10 @@ int vvm_net_grad_host(vvm_net_t *n, int nb, int h, int w, int periodic, const fl
                       const float *mask, float *grad, double *loss);
 /* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8) with the last gradient. */
 int vvm_net_adam(vvm_net_t *n, float lr);
+/* Tile height (2, 4 or 8 pixel rows; default 2) of the convolution kernels
+ * of every network in the process: a launch-shape knob like
+ * vvm_dev_set_shape, with identical results for every choice. */
+int vvm_net_set_conv_rows(int rows);
 
 /* ---- training data from the model's own runs ---------------------------
  * A timestep handle records one sample per solve (from its third solve on):
