@@ -424,6 +424,12 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
     // one row from prefetch slot q; with `refill` the slot is then loaded
     // again for row k + PFS
     auto row = [&](const int k, const int q, const bool refill) {
+        // The carry and the row's LDS values are read first, by every lane
+        // (one broadcast word), so their latency passes under the scan; the
+        // wait below only spins when the wave on the left has not yet
+        // published this row.
+        unsigned long long bits = src ? lds_load(src + k) : 0ull;
+        const double r = rate_l[k], tb = thbi[k], gk = ghost[k];
         double a[E], e[E];
         double b = 0.;
 #pragma unroll
@@ -470,37 +476,39 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
         const double ea = dppd<0x138, 0xF>(1., Pc[6]), eb = dppd<0x138, 0xF>(0., b);
         double y0;
         if (!src) {
-            y0 = ghost[k];  // ghost column, old as in the reference
+            y0 = gk;  // ghost column, old as in the reference
         } else {
-            unsigned long long bits = kEmpty;
-            if (lane == 0) {
-                int spins = 0;
-                while ((bits = lds_load(src + k)) == kEmpty && !gave_up) {
-                    if (++spins > kSpinLimit) {
-                        S.s[S_DIFF_ERR] = 1.;
-                        gave_up = true;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
+            // every lane holds the same word, so the loop is uniform
+            int spins = 0;
+            while (bits == kEmpty && !gave_up) {
+                if (++spins > kSpinLimit) {
+                    S.s[S_DIFF_ERR] = 1.;
+                    gave_up = true;
                 }
+                bits = lds_load(src + k);
             }
-            // lane 0 is the first active lane: a scalar broadcast
-            const int lo = __builtin_amdgcn_readfirstlane(int(bits & 0xffffffffull));
-            const int hi = __builtin_amdgcn_readfirstlane(int(bits >> 32));
-            y0 = __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+            y0 = __longlong_as_double(static_cast<long long>(bits));
         }
         double y = (lane == 0) ? y0 : ea * y0 + eb;
-        const double r = rate_l[k];
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int i = i0 + j;
             if (i > n) continue;
             y = a[j] * y + e[j];
             ynew[j] = y;
-            double o = y;
-            if (r != 0.) o = is_theta ? y - r * (y - thbi[k]) : y - r * y;
+        }
+        // publish the carry before the nudging and the store, which the
+        // wave on the right does not wait for
+        if (publish && i0 + E - 1 >= strip1 && i0 <= strip1)
+            lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(ynew[strip1 - i0])));
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int i = i0 + j;
+            if (i > n) continue;
+            const double yj = ynew[j];
+            double o = yj;
+            if (r != 0.) o = is_theta ? yj - r * (yj - tb) : yj - r * yj;
             f[ix(G, i, k)] = o;
-            if (i == strip1 && publish)
-                lds_store(dst + k, static_cast<unsigned long long>(__double_as_longlong(y)));
         }
 #pragma unroll
         for (int t = 0; t < 7; t++) Pc[t] = Pn[t];
