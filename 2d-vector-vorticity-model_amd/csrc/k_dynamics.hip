@@ -104,10 +104,14 @@ __device__ inline double thv(const Grid &G, const double *th, const double *qv, 
 // AB2 update of one history slot: writes d to slot sn (and to slot 0 on the
 // reference's step-0 start, Advection.cpp:70) and returns the old slot.
 __device__ inline double ab2_hist(double *const (&h)[2], size_t p, int step, double d) {
-    const int sn = (step + 1) % 2, so = step % 2;
-    h[sn][p] = d;
+    // the slot pointers are picked by selects, not by a run-time index into
+    // the kernel-argument array (which would read them through generic
+    // memory accesses)
+    const bool odd = (step & 1) != 0;
+    double *const hn = odd ? h[0] : h[1], *const ho = odd ? h[1] : h[0];
+    hn[p] = d;
     if (step == 0) h[0][p] = d;
-    return h[so][p];
+    return ho[p];
 }
 
 // ------------------------------------------------------------------ zeta

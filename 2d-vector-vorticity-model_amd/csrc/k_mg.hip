@@ -173,13 +173,17 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
 }
 
 // All levels from Lc down, in one workgroup with every array in LDS.
-template <typename T>
+template <typename T, bool GWS>
 __global__ void __launch_bounds__(1024) k_mg_coarse(CoarseParams<T> cp, const T *__restrict__ bin,
                                                     T *__restrict__ xout, T *ws) {
     extern __shared__ unsigned char lds_raw[];
-    // ws: global workspace of the same layout when the levels exceed LDS
-    // (__syncthreads orders global memory within the workgroup as well)
-    T *lds = ws ? ws : reinterpret_cast<T *>(lds_raw);
+    // GWS: the levels live in ws, a global workspace of the same layout,
+    // because they exceed LDS (__syncthreads orders global memory within the
+    // workgroup as well).  The two cases are separate instantiations so that
+    // the LDS one addresses LDS directly (ds_read/ds_write) rather than
+    // through generic pointers, whose flat accesses wait on both the memory
+    // and the LDS counters.
+    T *lds = GWS ? ws : reinterpret_cast<T *>(lds_raw);
     const T om = cp.omega;
     const int L = cp.nlev;
     auto B = [&](int l) { return lds + cp.offB[l]; };
@@ -409,7 +413,11 @@ void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doub
     }
     if (Lc < L) {
         T *xo = (Lc == 0) ? zA : H.lev[Lc].x;
-        hipLaunchKernelGGL(k_mg_coarse<T>, dim3(1), dim3(1024), H.coarse_lds, st, H.cp, bptr(Lc), xo, H.cws);
+        if (H.cws)
+            hipLaunchKernelGGL((k_mg_coarse<T, true>), dim3(1), dim3(1024), 0, st, H.cp, bptr(Lc), xo, H.cws);
+        else
+            hipLaunchKernelGGL((k_mg_coarse<T, false>), dim3(1), dim3(1024), H.coarse_lds, st, H.cp, bptr(Lc), xo,
+                               H.cws);
     }
     const bool dot_on_up = (Lc > 0);
     for (int l = std::min(Lc, L) - 1; l >= 0; l--) {
