@@ -39,13 +39,17 @@ __device__ inline T lap(const T *X, int W, int t, T cxj, T cuj, T cujm) {
     return cxj * (T(2) * c - X[t - 1] - X[t + 1]) + cuj * (c - X[t + W]) + cujm * (c - X[t - W]);
 }
 
-template <typename T>
+// CG: the right-hand side is the CG residual after its update, formed here
+// (cg, see CGFuse); otherwise b.
+template <typename T, bool CG>
 __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restrict__ b, T *__restrict__ x2out,
                                                  const T *__restrict__ cx, const T *__restrict__ cu,
                                                  const T *__restrict__ dinv, T omega, int Nc, int Mc,
-                                                 T *__restrict__ bc) {
+                                                 T *__restrict__ bc, CGFuse<T> cg) {
     __shared__ T Bs[DH * DW], X1[DH * DW], X2[DH * DW];
     __shared__ T Rx[DH], Ru[DH], Rd[DH];  // cx, cu, dinv of the tile's rows
+    T alpha = T(0);
+    if (CG) alpha = T(cg.sc[P_RZ] / block_total(cg.rs_pq, cg.npq));
     const int ic0 = blockIdx.x * CX, jc0 = 1 + blockIdx.y * CY;
     const int ic1 = min(ic0 + CX - 1, Nc - 1), jc1 = min(jc0 + CY - 1, Mc);
     const int fi0 = 2 * ic0 - 3, fj0 = 2 * jc0 - 3;
@@ -62,7 +66,11 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
         const int tj = t / W, ti = t % W;
         const int j = fj0 + tj, i = wrapi(fi0 + ti, N);
         const bool in = (j >= 1 && j <= M);
-        const T bv = in ? b[size_t(j) * N + i] : T(0);
+        T bv = T(0);
+        if (in) {
+            const size_t a = size_t(j) * N + i;
+            bv = CG ? cg.rin[a] - alpha * cg.q[a] : b[a];
+        }
         Bs[tj * DW + ti] = bv;
         X1[tj * DW + ti] = in ? omega * dinv[j] * bv : T(0);
     }
@@ -82,10 +90,29 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
     // 2jc0-1..2jc1 (and row M with the top coarse row)
     const int jtop = (jc1 == Mc) ? 2 * jc1 + 1 : 2 * jc1;
     const int ow = 2 * (ic1 - ic0) + 2, oh = jtop - (2 * jc0 - 1) + 1;
+    double part[1] = {0.};
     for (int t = threadIdx.x; t < ow * oh; t += blockDim.x) {
         const int oj = t / ow, oi = t % ow;
         const int i = 2 * ic0 + oi, j = 2 * jc0 - 1 + oj;
-        if (j >= 1 && j <= M) x2out[size_t(j) * N + i] = X2[(j - fj0) * DW + (i - fi0)];
+        if (j >= 1 && j <= M) {
+            const size_t a = size_t(j) * N + i;
+            const int s = (j - fj0) * DW + (i - fi0);
+            x2out[a] = X2[s];
+            if (CG) {  // the CG update on the points this workgroup owns
+                const T rn = Bs[s];
+                cg.rout[a] = rn;
+                cg.x[a] = cg.x[a] + alpha * cg.p[a];
+                const double wr = double(cg.wD[j]) * double(rn);
+                part[0] += wr * wr;
+            }
+        }
+    }
+    if (CG) {
+        block_sum<1>(part);  // two barriers: also orders the LDS reads below
+        if (threadIdx.x == 0) {
+            const int bid = blockIdx.x + gridDim.x * blockIdx.y;
+            __hip_atomic_store(cg.hpart + bid, part[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
     // residual on the restriction footprint, into X1
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
@@ -398,18 +425,30 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
 }
 
 template <typename T>
-void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init,
-               hipStream_t st) {
+int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStream_t st) {
+    if (H.coarse_from < 1 || H.nlev < 2) return 0;
+    const MGLevel<T> &F = H.lev[0];
+    const MGLevel<T> &C = H.lev[1];
+    const dim3 g((C.N + CX - 1) / CX, (C.M + CY - 1) / CY);
+    if (int(g.x * g.y) > max_blocks) return 0;
+    hipLaunchKernelGGL((k_mg_down<T, true>), g, dim3(256), 0, st, F.N, F.M, f.rin, zA, F.cx, F.cu, F.dinv,
+                       H.omega, C.N, C.M, C.b, f);
+    return int(g.x * g.y);
+}
+
+template <typename T>
+void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
+               bool skip_down0) {
     const int Lc = H.coarse_from, L = H.nlev;
     // level-l arrays: level 0 uses the CG's r (b) and z buffers
     auto bptr = [&](int l) -> const T * { return l == 0 ? r : H.lev[l].b; };
-    for (int l = 0; l < Lc && l < L - 1; l++) {
+    for (int l = skip_down0 ? 1 : 0; l < Lc && l < L - 1; l++) {
         const MGLevel<T> &F = H.lev[l];
         const MGLevel<T> &C = H.lev[l + 1];
         T *x2 = (l == 0) ? zA : H.lev[l].x;
         const dim3 g((C.N + CX - 1) / CX, (C.M + CY - 1) / CY);
-        hipLaunchKernelGGL(k_mg_down<T>, g, dim3(256), 0, st, F.N, F.M, bptr(l), x2, F.cx, F.cu, F.dinv, H.omega,
-                           C.N, C.M, C.b);
+        hipLaunchKernelGGL((k_mg_down<T, false>), g, dim3(256), 0, st, F.N, F.M, bptr(l), x2, F.cx, F.cu, F.dinv,
+                           H.omega, C.N, C.M, C.b, CGFuse<T>{});
     }
     if (Lc < L) {
         T *xo = (Lc == 0) ? zA : H.lev[Lc].x;
@@ -447,8 +486,10 @@ void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doub
 template int mg_plan<double>(MGHier<double> &, size_t);
 template int mg_plan<float>(MGHier<float> &, size_t);
 template void mg_vcycle<double>(MGHier<double> &, const double *, double *&, double *&, const RedSlot &, double *,
-                                bool, hipStream_t);
+                                bool, hipStream_t, bool);
 template void mg_vcycle<float>(MGHier<float> &, const float *, float *&, float *&, const RedSlot &, double *, bool,
-                               hipStream_t);
+                               hipStream_t, bool);
+template int mg_down0_cg<double>(MGHier<double> &, double *, const CGFuse<double> &, int, hipStream_t);
+template int mg_down0_cg<float>(MGHier<float> &, float *, const CGFuse<float> &, int, hipStream_t);
 
 }  // namespace vvmd

@@ -478,7 +478,8 @@ void free_hier(MGHier<T> &H) {
 // the iteration count; *rn2 gets the last residual norm^2 seen.
 template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
-             double *hs_dev, hipEvent_t ev, hipStream_t st, double rn_start, double *rn2, bool *conv) {
+             double *hs_dev, const double *hpart, double *hpart_dev, hipEvent_t ev, hipStream_t st,
+             double rn_start, double *rn2, bool *conv) {
     const int nb = cg_blocks(P.N, P.M);
     const dim3 b(LX, LY);
     // p.q partials in the last row of the reduction slots, which the update
@@ -495,10 +496,18 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
         hipLaunchKernelGGL(k_pcg_pq<T>, dim3(nb), b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
                            P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1));
         std::swap(P.pA, P.pB);
-        hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.wD, P.sc, rs_pq,
-                           nb, rs);
-        // ||w r||^2 into sc and straight into the pinned host word hs
-        finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
+        // the update inside the level-0 down kernel when the hierarchy has
+        // one (its workgroups leave ||w r||^2 partials in host memory), else
+        // its own kernel and a reduction into the host word hs
+        const CGFuse<T> fz{P.x, P.r, P.rB, P.pA, P.q, P.wD, P.sc, rs_pq, nb, hpart_dev};
+        const int nfused = hpart_dev ? mg_down0_cg(P.mg, P.zA, fz, rs.max_blocks, st) : 0;
+        if (nfused) {
+            std::swap(P.r, P.rB);
+        } else {
+            hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.wD, P.sc,
+                               rs_pq, nb, rs);
+            finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
+        }
         hipEventRecord(ev, st);
         bool ahead = it < max_iter;
         if (ahead && !stall_stop) {
@@ -507,15 +516,21 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             const double q = (prev2 > 0.) ? std::min(prev / prev2, 1.) : 1e-2;
             if (prev * q <= target) ahead = false;
         }
-        if (ahead) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
+        if (ahead) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
         hipEventSynchronize(ev);
-        const double rn = *static_cast<volatile double *>(hs);
+        double rn = 0.;
+        if (nfused) {
+            const volatile double *hp = hpart;
+            for (int k = 0; k < nfused; k++) rn += hp[k];  // fixed order: deterministic
+        } else {
+            rn = *static_cast<volatile double *>(hs);
+        }
         *rn2 = rn;
         if (!(rn > target)) {
             *conv = true;
             break;
         }
-        if (!ahead && it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st);
+        if (!ahead && it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
         prev2 = prev;
         prev = rn;
         if (stall_stop) {
@@ -550,6 +565,7 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         if (!build_hier(P.mg, N, M, cx, cu)) return VVM_ERR_NOMEM;
         P.x = dalloc<double>(n);
         P.r = dalloc<double>(n);
+        P.rB = dalloc<double>(n);
         P.zA = dalloc<double>(n);
         P.zB = dalloc<double>(n);
         P.pA = dalloc<double>(n);
@@ -557,7 +573,8 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         P.q = dalloc<double>(n);
         P.wD = dalloc<double>(M + 2);
         P.sc = dalloc<double>(NPSCAL);
-        if (!P.x || !P.r || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc) return VVM_ERR_NOMEM;
+        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc)
+            return VVM_ERR_NOMEM;
         hipMemcpy(P.wD, wd.data(), sizeof(double) * (M + 2), hipMemcpyHostToDevice);
     }
     have32 = with_fp32;
@@ -568,6 +585,7 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         if (!build_hier(P.mg, N, M, cx, cu)) return VVM_ERR_NOMEM;
         P.x = dalloc<float>(n);
         P.r = dalloc<float>(n);
+        P.rB = dalloc<float>(n);
         P.zA = dalloc<float>(n);
         P.zB = dalloc<float>(n);
         P.pA = dalloc<float>(n);
@@ -575,7 +593,8 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         P.q = dalloc<float>(n);
         P.wD = dalloc<float>(M + 2);
         P.sc = dalloc<double>(NPSCAL);
-        if (!P.x || !P.r || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc) return VVM_ERR_NOMEM;
+        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc)
+            return VVM_ERR_NOMEM;
         std::vector<float> wf(wd.begin(), wd.end());
         hipMemcpy(P.wD, wf.data(), sizeof(float) * (M + 2), hipMemcpyHostToDevice);
         r64 = dalloc<double>(n);
@@ -591,6 +610,11 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
         return VVM_ERR_NOMEM;
     if (hipHostGetDevicePointer(reinterpret_cast<void **>(&hscal_dev), hscal, 0) != hipSuccess) return VVM_ERR_HIP;
+    // per-workgroup residual norms of the fused CG update, read by the host
+    if (hipHostMalloc(reinterpret_cast<void **>(&hpart), sizeof(double) * size_t(rs.max_blocks),
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+        return VVM_ERR_NOMEM;
+    if (hipHostGetDevicePointer(reinterpret_cast<void **>(&hpart_dev), hpart, 0) != hipSuccess) return VVM_ERR_HIP;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return VVM_ERR_HIP;
     return VVM_OK;
 }
@@ -598,8 +622,8 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
 template <typename T>
 static void free_pcg(PCG<T> &P) {
     free_hier(P.mg);
-    for (void *p : {(void *)P.x, (void *)P.r, (void *)P.zA, (void *)P.zB, (void *)P.pA, (void *)P.pB, (void *)P.q,
-                    (void *)P.wD, (void *)P.sc})
+    for (void *p : {(void *)P.x, (void *)P.r, (void *)P.rB, (void *)P.zA, (void *)P.zB, (void *)P.pA,
+                    (void *)P.pB, (void *)P.q, (void *)P.wD, (void *)P.sc})
         if (p) hipFree(p);
     P = PCG<T>();
 }
@@ -619,6 +643,8 @@ void Poisson::destroy() {
     if (hscal) hipHostFree(hscal);
     hscal = nullptr;
     hscal_dev = nullptr;
+    if (hpart) hipHostFree(hpart);
+    hpart = hpart_dev = nullptr;
     if (ev) hipEventDestroy(ev);
     ev = nullptr;
 }
@@ -698,7 +724,7 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     }
     double last = rn2;
     bool conv = false;
-    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, ev, st, rn2, &last, &conv);
+    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, rn2, &last, &conv);
     R.rel = std::sqrt(last / bn2);
     R.converged = conv;
     return R;
@@ -748,7 +774,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         const double r0 = rn2;
         double last = r0;
         bool conv = false;
-        R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, ev, st, r0, &last,
+        R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, r0, &last,
                             &conv);
         hipLaunchKernelGGL(k_add_correction, g, b, 0, st, N, M, D.x, F.x);
     }

@@ -56,8 +56,30 @@ int mg_plan(MGHier<T> &H, size_t lds_budget);
 
 // One V-cycle on r (level-0 right-hand side) into zA (zA/zB are swapped as
 // the result moves between buffers); also forms r.z and the CG beta in sc
-// (init: only r.z).
+// (init: only r.z).  With skip_down0 the level-0 down step has already run
+// (mg_down0_cg).
 template <typename T>
-void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st);
+void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
+               bool skip_down0 = false);
+
+// The CG update fused into the level-0 down step: alpha = sc[P_RZ] / p.q
+// (p.q from the npq partials in rs_pq), rout = rin - alpha q and x += alpha p
+// on the points each workgroup owns, the smoothing and restriction on the updated
+// r, and ||w r||^2 per workgroup into hpart (host-mapped memory; the host
+// adds the nblocks entries).  Returns nblocks, or 0 when the hierarchy has
+// no tiled level 0 (the caller then runs the update separately).
+template <typename T>
+struct CGFuse {
+    T *x;
+    const T *rin;  // r before the update (read with halos by every workgroup)
+    T *rout;       // r after it (another buffer: the reads above must see the old r)
+    const T *p, *q, *wD;
+    const double *sc;
+    RedSlot rs_pq;
+    int npq;
+    double *hpart;
+};
+template <typename T>
+int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStream_t st);
 
 }  // namespace vvmd

@@ -24,6 +24,7 @@ struct PCG {
     int N = 0, M = 0;
     MGHier<T> mg;
     T *x = nullptr, *r = nullptr, *zA = nullptr, *zB = nullptr, *pA = nullptr, *pB = nullptr, *q = nullptr;
+    T *rB = nullptr;       // the other residual buffer of the fused update (swapped with r)
     T *wD = nullptr;       // row weights rhow[k] for ||D r||, [M+2]
     double *sc = nullptr;  // NPSCAL device scalars
 };
@@ -100,6 +101,7 @@ class Poisson {
     double *r64 = nullptr;     // FP64 residual of the refinement loop
     double *hscal = nullptr;   // pinned host copy of scalars
     double *hscal_dev = nullptr;  // the same memory as the device sees it
+    double *hpart = nullptr, *hpart_dev = nullptr;  // per-workgroup norms (mapped host memory)
     RedSlot red;
     float *gfeat = nullptr;    // [3][M][N] features of the learned guess
     double *gsc = nullptr;     // sums of squares of the three features
