@@ -112,7 +112,10 @@ __device__ inline void block_partials(double (&v)[NV], const RedSlot &rs) {
 }
 
 // Scalars of the preconditioned CG (doubles in its scalar block).
-enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, NPSCAL = 8 };
+// P_RZ / P_RZ2: r.z of odd / even CG iterations (two slots, so that the
+// kernel forming the new value never overwrites the one its other
+// workgroups still read).
+enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, P_RZ2, NPSCAL = 8 };
 
 // What k_finalize does with the totals.
 enum RedMode {

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
     __shared__ T Bs[DH * DW], X1[DH * DW], X2[DH * DW];
     __shared__ T Rx[DH], Ru[DH], Rd[DH];  // cx, cu, dinv of the tile's rows
     T alpha = T(0);
-    if (CG) alpha = T(cg.sc[P_RZ] / block_total(cg.rs_pq, cg.npq));
+    if (CG) alpha = T(cg.sc[cg.rz] / block_total(cg.rs_pq, cg.npq));
     const int ic0 = blockIdx.x * CX, jc0 = 1 + blockIdx.y * CY;
     const int ic1 = min(ic0 + CX - 1, Nc - 1), jc1 = min(jc0 + CY - 1, Mc);
     const int fi0 = 2 * ic0 - 3, fj0 = 2 * jc0 - 3;
@@ -437,8 +437,9 @@ int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStre
 }
 
 template <typename T>
-void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
-               bool skip_down0) {
+int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
+              bool skip_down0) {
+    int nrz = 0;
     const int Lc = H.coarse_from, L = H.nlev;
     // level-l arrays: level 0 uses the CG's r (b) and z buffers
     auto bptr = [&](int l) -> const T * { return l == 0 ? r : H.lev[l].b; };
@@ -469,7 +470,8 @@ void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doub
                            (l + 1 < L) ? H.lev[l + 1].N : 1, xc, rs, int(l == 0));
         if (l == 0) {
             std::swap(zA, zB);
-            finalize<1>(rs, int(g.x * g.y), sc, init ? RED_RZ_INIT : RED_BETA, 0, st);
+            nrz = int(g.x * g.y);
+            if (init) finalize<1>(rs, nrz, sc, RED_RZ_INIT, 0, st);
         } else {
             std::swap(F.x, F.t);
         }
@@ -479,16 +481,18 @@ void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doub
         const int n = (F.M + 2) * F.N;
         const int nb = std::min(1024, (n + 255) / 256);
         hipLaunchKernelGGL(k_dot<T>, dim3(nb), dim3(256), 0, st, n, r, zA, rs);
-        finalize<1>(rs, nb, sc, init ? RED_RZ_INIT : RED_BETA, 0, st);
+        nrz = nb;
+        if (init) finalize<1>(rs, nb, sc, RED_RZ_INIT, 0, st);
     }
+    return nrz;
 }
 
 template int mg_plan<double>(MGHier<double> &, size_t);
 template int mg_plan<float>(MGHier<float> &, size_t);
-template void mg_vcycle<double>(MGHier<double> &, const double *, double *&, double *&, const RedSlot &, double *,
-                                bool, hipStream_t, bool);
-template void mg_vcycle<float>(MGHier<float> &, const float *, float *&, float *&, const RedSlot &, double *, bool,
-                               hipStream_t, bool);
+template int mg_vcycle<double>(MGHier<double> &, const double *, double *&, double *&, const RedSlot &, double *,
+                               bool, hipStream_t, bool);
+template int mg_vcycle<float>(MGHier<float> &, const float *, float *&, float *&, const RedSlot &, double *, bool,
+                              hipStream_t, bool);
 template int mg_down0_cg<double>(MGHier<double> &, double *, const CGFuse<double> &, int, hipStream_t);
 template int mg_down0_cg<float>(MGHier<float> &, float *, const CGFuse<float> &, int, hipStream_t);
 

@@ -42,12 +42,20 @@ constexpr int kCGBlocks = 1024;
 inline int cg_blocks(int N, int M) { return std::min(kCGBlocks, int(nblocks(lgrid(N, M)))); }
 
 // p = z + beta p (beta = 0 on the first iteration), q = L p, partials of p.q
-// into rs_pq.
+// into rs_pq.  After the first iteration every workgroup adds the n_rz
+// partials of r.z the preconditioner left in rs_rz, beta = r.z / sc[old];
+// workgroup 0 stores r.z in sc[fresh] for the update.
 template <typename T>
 __global__ void __launch_bounds__(LX *LY) k_pcg_pq(int N, int M, const T *z, const T *pin, T *pout, T *q, const T *cx,
-                                                   const T *cu, double *sc, RedSlot rs_pq, int first) {
+                                                   const T *cu, double *sc, RedSlot rs_pq, int first, RedSlot rs_rz,
+                                                   int n_rz, int old, int fresh) {
     const int tiles_x = (N + LX - 1) / LX, ntiles = tiles_x * ((M + LY - 1) / LY);
-    const T beta = first ? T(0) : T(sc[P_BETA]);
+    T beta = T(0);
+    if (!first) {
+        const double rz = block_total(rs_rz, n_rz);
+        beta = T(rz / sc[old]);
+        if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) sc[fresh] = rz;
+    }
     double part[1] = {0.};
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int i = (t % tiles_x) * LX + threadIdx.x, j = 1 + (t / tiles_x) * LY + threadIdx.y;
@@ -66,14 +74,15 @@ __global__ void __launch_bounds__(LX *LY) k_pcg_pq(int N, int M, const T *z, con
     block_partials<1>(part, rs_pq);
 }
 
-// alpha = rz / p.q (p.q from the npq partials of k_pcg_pq), x += alpha p,
+// alpha = sc[rz] / p.q (p.q from the npq partials of k_pcg_pq), x += alpha p,
 // r -= alpha q, and ||w r||^2 with the row weights w (rhow: the reference
 // system's residual) as partials in rs.
 template <typename T>
 __global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r, const T *p, const T *q,
-                                                       const T *wD, double *sc, RedSlot rs_pq, int npq, RedSlot rs) {
+                                                       const T *wD, double *sc, int rz, RedSlot rs_pq, int npq,
+                                                       RedSlot rs) {
     const int tiles_x = (N + LX - 1) / LX, ntiles = tiles_x * ((M + LY - 1) / LY);
-    const T alpha = T(sc[P_RZ] / block_total(rs_pq, npq));
+    const T alpha = T(sc[rz] / block_total(rs_pq, npq));
     double part[1] = {0.};
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
         const int i = (t % tiles_x) * LX + threadIdx.x, j = 1 + (t / tiles_x) * LY + threadIdx.y;
@@ -485,7 +494,8 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
     // p.q partials in the last row of the reduction slots, which the update
     // kernel reads while it writes its own partials to row 0
     const RedSlot rs_pq{rs.partial + size_t(7) * rs.max_blocks, rs.max_blocks};
-    mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st);
+    mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st);  // r.z into sc[P_RZ]
+    int n_rz = 0;  // r.z partials of the last V-cycle, summed by the next pq
     int it = 0;
     double best = 1e300;
     int stall = 0;
@@ -493,19 +503,20 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
     *conv = false;
     while (it < max_iter) {
         it++;
+        const int rz_old = (it & 1) ? P_RZ2 : P_RZ, rz_now = (it & 1) ? P_RZ : P_RZ2;  // it = 1: P_RZ
         hipLaunchKernelGGL(k_pcg_pq<T>, dim3(nb), b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
-                           P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1));
+                           P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1), rs, n_rz, rz_old, rz_now);
         std::swap(P.pA, P.pB);
         // the update inside the level-0 down kernel when the hierarchy has
         // one (its workgroups leave ||w r||^2 partials in host memory), else
         // its own kernel and a reduction into the host word hs
-        const CGFuse<T> fz{P.x, P.r, P.rB, P.pA, P.q, P.wD, P.sc, rs_pq, nb, hpart_dev};
+        const CGFuse<T> fz{P.x, P.r, P.rB, P.pA, P.q, P.wD, P.sc, rz_now, rs_pq, nb, hpart_dev};
         const int nfused = hpart_dev ? mg_down0_cg(P.mg, P.zA, fz, rs.max_blocks, st) : 0;
         if (nfused) {
             std::swap(P.r, P.rB);
         } else {
             hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.wD, P.sc,
-                               rs_pq, nb, rs);
+                               rz_now, rs_pq, nb, rs);
             finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
         }
         hipEventRecord(ev, st);
@@ -516,7 +527,7 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             const double q = (prev2 > 0.) ? std::min(prev / prev2, 1.) : 1e-2;
             if (prev * q <= target) ahead = false;
         }
-        if (ahead) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
+        if (ahead) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
         hipEventSynchronize(ev);
         double rn = 0.;
         if (nfused) {
@@ -530,7 +541,7 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             *conv = true;
             break;
         }
-        if (!ahead && it < max_iter) mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
+        if (!ahead && it < max_iter) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
         prev2 = prev;
         prev = rn;
         if (stall_stop) {

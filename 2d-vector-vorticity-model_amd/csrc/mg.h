@@ -55,14 +55,15 @@ template <typename T>
 int mg_plan(MGHier<T> &H, size_t lds_budget);
 
 // One V-cycle on r (level-0 right-hand side) into zA (zA/zB are swapped as
-// the result moves between buffers); also forms r.z and the CG beta in sc
-// (init: only r.z).  With skip_down0 the level-0 down step has already run
-// (mg_down0_cg).
+// the result moves between buffers).  With init it also stores r.z in
+// sc[P_RZ]; otherwise it leaves the r.z partials in row 0 of rs for the next
+// k_pcg_pq and returns their count.  With skip_down0 the level-0 down step
+// has already run (mg_down0_cg).
 template <typename T>
-void mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
-               bool skip_down0 = false);
+int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
+              bool skip_down0 = false);
 
-// The CG update fused into the level-0 down step: alpha = sc[P_RZ] / p.q
+// The CG update fused into the level-0 down step: alpha = sc[rz] / p.q
 // (p.q from the npq partials in rs_pq), rout = rin - alpha q and x += alpha p
 // on the points each workgroup owns, the smoothing and restriction on the updated
 // r, and ||w r||^2 per workgroup into hpart (host-mapped memory; the host
@@ -75,6 +76,7 @@ struct CGFuse {
     T *rout;       // r after it (another buffer: the reads above must see the old r)
     const T *p, *q, *wD;
     const double *sc;
+    int rz;  // slot of r.z in sc
     RedSlot rs_pq;
     int npq;
     double *hpart;
