@@ -670,13 +670,13 @@ int Poisson::replan(size_t budget) {
 
 void Poisson::rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st) {
     hipLaunchKernelGGL(k_calw_rhs, lgrid(N, M), dim3(LX, LY), 0, st, G, zetap, rhow_dev, f, pcg64.sc, red);
-    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st);
+    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st, -1, hscal_dev + P_BNORM2);
 }
 
 void Poisson::rhs_from_b(const double *b, hipStream_t st) {
     // the FP64 row weights wD[j] are rhow[j+1]
     hipLaunchKernelGGL(k_rhs_from_b, lgrid(N, M), dim3(LX, LY), 0, st, N, M, b, pcg64.wD, f, pcg64.sc, red);
-    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st);
+    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st, -1, hscal_dev + P_BNORM2);
 }
 
 void Poisson::x_to_ref(double *xref, hipStream_t st) {
@@ -716,8 +716,8 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     const dim3 g = lgrid(N, M), b(LX, LY);
     hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, P.x, P.r, P.mg.lev[0].cx, P.mg.lev[0].cu,
                        P.wD, P.sc, red, int(P_RNORM2), int(P_RNORM2_0));
-    finalize<1>(red, nblocks(g), P.sc, RED_STORE, P_RNORM2, st, P_RNORM2_0);
-    hipMemcpyAsync(hscal, P.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
+    // the norms land in mapped host memory (hscal) directly: no copy in the stream
+    finalize<1>(red, nblocks(g), P.sc, RED_STORE, P_RNORM2, st, P_RNORM2_0, hscal_dev + P_RNORM2);
     hipStreamSynchronize(st);
     const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
     if (bn2 == 0.) {
@@ -754,8 +754,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
     for (int outer = 0; outer <= max_outer; outer++) {
         hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, D.x, r64, D.mg.lev[0].cx,
                            D.mg.lev[0].cu, D.wD, D.sc, red, int(P_RNORM2), -1);
-        finalize<1>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st);
-        hipMemcpyAsync(hscal, D.sc, NPSCAL * sizeof(double), hipMemcpyDeviceToHost, st);
+        finalize<1>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st, -1, hscal_dev + P_RNORM2);
         hipStreamSynchronize(st);
         bn2 = hscal[P_BNORM2];
         const double rn2 = hscal[P_RNORM2];
