@@ -49,6 +49,24 @@ KERNEL_BYTES_PER_POINT = {
 }
 
 
+# Algorithmic bytes of a whole step (the same rule): the non-solver kernels
+# above plus the solve, per PCG iteration and unknown ~220 B in FP64 (CG
+# vectors 88 B + a V(2,2) cycle ~4/3 x 100 B) and ~110 B in FP32, plus
+# ~100 B per unknown for each FP64 refinement (residual, conversion,
+# correction).
+STEP_NONSOLVER_BYTES_PER_POINT = sum(KERNEL_BYTES_PER_POINT.values()) + 64.0  # the second scale pass
+SOLVER_BYTES_PER_ITER = {"f64": 220.0, "f32": 110.0}
+REFINE_BYTES = 100.0
+
+
+def step_roofline(ms_per_step: float, iters: float, refine: float, fp32: bool, pts: int, unknowns: int) -> dict:
+    """The whole step as a fraction of HBM peak, from algorithmic bytes."""
+    per_iter = SOLVER_BYTES_PER_ITER["f32" if fp32 else "f64"]
+    b = STEP_NONSOLVER_BYTES_PER_POINT * pts + (iters * per_iter + refine * REFINE_BYTES) * unknowns
+    gbs = b / (ms_per_step * 1e-3) / 1e9
+    return {"bytes_per_step": b, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS}
+
+
 # kernels behind each timed phase (rows of the committed traffic table)
 KERNEL_NAMES = {
     "diffusion": ["k_diff_coeffs", "k_diff_strips"], "thermo4": ["k_thermo4"], "pzeta": ["k_pzeta"],
@@ -296,6 +314,9 @@ def main():
                                    "diffusion": 1e3 * stats.t_diffusion},
             "kernel_avg_us": 1e6 * kavg, "kernel_launches": kn,
         }
+        results[name]["step_roofline"] = step_roofline(
+            results[name]["ms_per_step"], results[name]["solve_iters_per_step"], float(stats.refine_last),
+            paths[name]["solver_mode"] == V.VVM_SOLVER_MIXED, pts, (base.cfg.nx - 2) * (base.cfg.nz - 3))
         log(f"[bench] {name}: {results[name]['value']:.4g} grid-point-steps/s, "
             f"{results[name]['ms_per_step']:.3f} ms/step, {results[name]['solve_iters_per_step']:.2f} iters/step")
 
@@ -322,7 +343,7 @@ def main():
             out[f"speedup_{p}_vs_fp64"] = results[p]["value"] / results["fp64_plain"]["value"]
             a, b = finals[p], finals["fp64_plain"]
             out[f"{p}_vs_fp64_rel_l2"] = {f: float(np.linalg.norm(a[f] - b[f]) / max(np.linalg.norm(b[f]), 1e-300))
-                                          for f in ("zeta", "th", "qv", "u", "w")}
+                                          for f in ("zeta", "th", "qv", "qc", "qr", "u", "w")}
     if "mixed_learned" in results:
         out["learned_guess"] = learned_info
     if not args.no_cpu_baseline:
