@@ -9,12 +9,14 @@
 //
 // Forward / backward-data kernel (k_conv): a workgroup of 4 wavefronts
 // computes a 64 x 2, 4 or 8 pixel tile; each wavefront owns one to four
-// v_mfma_f32_32x32x16_bf16 output blocks of 32 pixels x 32 output channels,
+// v_mfma_f32_32x32x16_bf16 output blocks of 32 output channels x 32 pixels,
 // and walks K = 9 taps x 32 input channels in 18 steps of 16.  The input
-// tile with its one-pixel halo is staged once in LDS.  The
-// B operand (weights) is pre-packed in the exact lane order of the
-// instruction, so a lane fetches its 8 weights as one 16-byte load and
-// uses them for its four blocks.
+// tile with its one-pixel halo is staged once in LDS and read as the B
+// operand.  The A operand (weights) is pre-packed in the exact lane order
+// of the instruction, so a lane fetches its 8 weights as one 16-byte load
+// and uses them for all its blocks.  With channels as the rows of the
+// result, a lane ends with 16 channels of one pixel and stores them as four
+// 8-byte runs.
 //
 // Weight-gradient kernel (k_conv_wgrad): one wavefront per tap; the GEMM
 // runs over pixels (K) with the input tile and the output gradient staged
@@ -29,6 +31,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TW = 64, TH = 2;                 // weight-gradient tile (columns x rows)
 constexpr int HW = TW + 2, HH = TH + 2;        // its haloed tile
@@ -39,16 +42,25 @@ constexpr int PXS = 40;                        // LDS pixel stride in bf16 (80 b
 constexpr float GC = 0.7978845608028654f;      // sqrt(2/pi)
 constexpr float GA = 0.044715f;
 
-// tanh(u) = 1 - 2 / (1 + e^{2u}) on the hardware exponential: a few
-// instructions instead of the library's full-accuracy tanhf, which made the
-// GELU epilogue the costliest part of a convolution.  Its error (~1e-7
-// absolute) is far below the bf16 rounding the activations get next; both
-// ends saturate correctly (e^{2u} -> inf gives 1, -> 0 gives -1).
-__device__ inline float tanh_fast(float u) { return 1.f - 2.f / (1.f + __expf(2.f * u)); }
-__device__ inline float gelu(float z) { return 0.5f * z * (1.f + tanh_fast(GC * (z + GA * z * z * z))); }
+// GELU (tanh form) written as z * sigmoid(2u), u = sqrt(2/pi) (z + 0.044715
+// z^3), since 0.5 (1 + tanh u) = 1 / (1 + e^{-2u}): one hardware exp2 and
+// one hardware reciprocal (v_exp_f32, v_rcp_f32, ~1 ulp each) and three
+// multiply-adds.  The epilogue evaluates it for every output channel of
+// every pixel, which made the library's full-accuracy tanhf and IEEE
+// division the costliest part of a convolution; the ~1e-7 relative error
+// here is far below the bf16 rounding the activations get next.  Both ends
+// saturate: e -> inf gives z * 0, e -> 0 gives z.
+constexpr float LOG2E = 1.4426950408889634f;
+constexpr float S1 = -2.f * GC * LOG2E, S3 = S1 * GA;  // -2u log2(e) = z (S1 + S3 z^2)
+__device__ inline float sigm2u(float z) {
+    const float e = __builtin_amdgcn_exp2f(z * __builtin_fmaf(S3, z * z, S1));
+    return __builtin_amdgcn_rcpf(1.f + e);
+}
+__device__ inline float gelu(float z) { return z * sigm2u(z); }
+// d/dz [z s(z)] with s = sigmoid(2u): s + z s (1 - s) 2u'
 __device__ inline float gelu_d(float z) {
-    const float t = tanh_fast(GC * (z + GA * z * z * z));
-    return 0.5f * (1.f + t) + 0.5f * z * (1.f - t * t) * GC * (1.f + 3.f * GA * z * z);
+    const float sg = sigm2u(z);
+    return sg + z * sg * (1.f - sg) * (2.f * GC) * __builtin_fmaf(3.f * GA, z * z, 1.f);
 }
 
 __device__ inline __bf16 to_bf16(float x) { return static_cast<__bf16>(x); }
@@ -133,37 +145,157 @@ __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict
             const int gq = BPW * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
             const int px = (ty + 1 + dy) * FHW + (cb + i + 1 + dx);
             const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc[q], 0, 0, 0);
+            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a, acc[q], 0, 0, 0);
         }
     }
-    // D[m][n]: n = lane % 32 (output channel), m = 8 (r / 4) + 4 (lane / 32) + r % 4 (pixel)
-    const int co = lane & 31;
+    // weights as the A operand, pixels as B: D[m][n] with n = lane % 32 the
+    // pixel and m = 8 (r / 4) + 4 (lane / 32) + r % 4 the output channel, so
+    // a lane holds four runs of 4 consecutive channels of one pixel
+    float bv[16];
+    if (MODE == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) bv[r] = bias[8 * (r >> 2) + 4 * gk + (r & 3)];
+    }
 #pragma unroll
     for (int q = 0; q < BPW; q++) {
-        const int gq = BPW * wv + q, y = y0 + (gq >> 1), cb = (gq & 1) * 32;
-        if (y >= g.h) continue;
+        const int gq = BPW * wv + q, y = y0 + (gq >> 1), x = x0 + (gq & 1) * 32 + i;
+        if (y >= g.h || x >= g.w) continue;
+        const long p = (long(b) * g.h + y) * g.w + x;
+        if (MODE == 0 || MODE == 2) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int m = 8 * (r >> 2) + 4 * gk + (r & 3);
-            const int x = x0 + cb + m;
-            if (x >= g.w) continue;
-            const long p = (long(b) * g.h + y) * g.w + x;
-            if (MODE == 0) {
-                const float z = acc[q][r] + bias[co];
-                if (outZ) outZ[p * NET_C + co] = to_bf16(z);  // kept for training only
-                outA[p * NET_C + co] = to_bf16(gelu(z));
-            } else if (MODE == 1) {
-                if (co == 0) outF[p] = acc[q][r] + bias[0];
-            } else if (MODE == 3) {
-                if (co == 0) {
-#pragma clang fp contract(off)
-                    const float o = acc[q][r] + bias[0];
-                    const size_t a = size_t(y + 1) * ap.N + x;
-                    ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+            for (int j = 0; j < 4; j++) {
+                const long e = p * NET_C + 8 * j + 4 * gk;  // channels 8 j + 4 gk .. + 3
+                bf16x4 o;
+                if (MODE == 0) {
+                    bf16x4 zq;
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const float z = acc[q][4 * j + c] + bv[4 * j + c];
+                        zq[c] = to_bf16(z);
+                        o[c] = to_bf16(gelu(z));
+                    }
+                    if (outZ) *reinterpret_cast<bf16x4 *>(outZ + e) = zq;  // kept for training only
+                } else {
+                    const bf16x4 zb = *reinterpret_cast<const bf16x4 *>(zbelow + e);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) o[c] = to_bf16(acc[q][4 * j + c] * gelu_d(from_bf16(zb[c])));
                 }
+                *reinterpret_cast<bf16x4 *>(outA + e) = o;
+            }
+        } else if (gk == 0) {  // channel 0 is m = 0: register 0 of lanes 0..31
+            if (MODE == 1) {
+                outF[p] = acc[q][0] + bias[0];
             } else {
-                const float dz = acc[q][r] * gelu_d(from_bf16(zbelow[p * NET_C + co]));
-                outA[p * NET_C + co] = to_bf16(dz);
+#pragma clang fp contract(off)
+                const float o = acc[q][0] + bias[0];
+                const size_t a = size_t(y + 1) * ap.N + x;
+                ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+            }
+        }
+    }
+}
+
+// Two hidden layers in one pass (inference): layer a on a 64 x 6 pixel
+// region kept in LDS, then layer b on the 62 x 4 pixel tile inside it, so
+// the activations between them never go to memory.  The output tile is 62
+// columns wide so that layer a's region is exactly two 32-pixel MFMA blocks
+// a row (12 blocks, 3 a wavefront); layer b computes 64 columns and keeps
+// 62 (8 blocks, 2 a wavefront).  Layer a's values outside the image are the
+// zero padding of layer b's input.
+constexpr int PTW = 62, PTH = 4;                  // output tile
+constexpr int PIW = PTW + 4, PIH = PTH + 4;       // input tile (two-pixel halo)
+constexpr int PMW = PTW + 4, PMH = PTH + 2;       // layer-a region (64 columns used, 2 spare)
+
+__global__ void __launch_bounds__(256) k_conv_pair(NetGeo g, const __bf16 *__restrict__ in,
+                                                   const bf16x8 *__restrict__ wa, const float *__restrict__ ba,
+                                                   const bf16x8 *__restrict__ wb, const float *__restrict__ bb,
+                                                   __bf16 *__restrict__ out) {
+    __shared__ __attribute__((aligned(16))) __bf16 tin[PIH * PIW * PXS];
+    __shared__ __attribute__((aligned(16))) __bf16 mid[PMH * PMW * PXS];
+    const int b = blockIdx.z, y0 = blockIdx.y * PTH, x0 = blockIdx.x * PTW;
+    bf16x8 wr[18];  // layer a's weight fragments, requested before the staging
+#pragma unroll
+    for (int s = 0; s < 18; s++) wr[s] = wa[s * 64 + (threadIdx.x & 63)];
+    // input rows y0-2 .., columns x0-2 .. (halo_src takes the tile origin one
+    // pixel inside the halo)
+    for (int c = threadIdx.x; c < PIH * PIW * 4; c += blockDim.x) {
+        const int px = c >> 2, part = c & 3;
+        const long src = halo_src(g, b, y0 - 1, x0 - 1, px / PIW, px % PIW);
+        bf16x8 v = {};
+        if (src >= 0) v = *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8);
+        *reinterpret_cast<bf16x8 *>(tin + px * PXS + part * 8) = v;
+    }
+    for (int c = threadIdx.x; c < PMH * 2 * 4; c += blockDim.x) {  // spare columns 64, 65 of mid: zero
+        const int row = c / 8, col = 64 + (c / 4) % 2, part = c % 4;
+        *reinterpret_cast<bf16x8 *>(mid + (row * PMW + col) * PXS + part * 8) = bf16x8{};
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lane & 31, gk = lane >> 5;
+    {  // layer a: blocks 3 wv .. 3 wv + 2 of the 12 (row gq / 2, half gq % 2)
+        f32x16 acc[3] = {};
+#pragma unroll
+        for (int s = 0; s < 18; s++) {
+            const int t = s >> 1, hh = s & 1;
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 w = wr[s];
+#pragma unroll
+            for (int q = 0; q < 3; q++) {
+                const int gq = 3 * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
+                const int px = (ty + 1 + dy) * PIW + (cb + i + 1 + dx);
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tin + px * PXS + hh * 16 + gk * 8);
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a, acc[q], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < 18; s++) wr[s] = wb[s * 64 + lane];  // layer b's, during the epilogue
+        float bv[16];  // D[channel][pixel] as in k_conv
+#pragma unroll
+        for (int r = 0; r < 16; r++) bv[r] = ba[8 * (r >> 2) + 4 * gk + (r & 3)];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int gq = 3 * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
+            const int y = y0 - 1 + ty, x = x0 - 1 + cb + i;
+            const bool in_img = y >= 0 && y < g.h && (g.periodic || (x >= 0 && x < g.w));
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                bf16x4 o;
+#pragma unroll
+                for (int c = 0; c < 4; c++) o[c] = to_bf16(in_img ? gelu(acc[q][4 * j + c] + bv[4 * j + c]) : 0.f);
+                *reinterpret_cast<bf16x4 *>(mid + (ty * PMW + cb + i) * PXS + 8 * j + 4 * gk) = o;
+            }
+        }
+    }
+    __syncthreads();
+    {  // layer b: blocks 2 wv, 2 wv + 1 of the 8
+        f32x16 acc[2] = {};
+#pragma unroll
+        for (int s = 0; s < 18; s++) {
+            const int t = s >> 1, hh = s & 1;
+            const int dy = t / 3 - 1, dx = t % 3 - 1;
+            const bf16x8 w = wr[s];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const int gq = 2 * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
+                const int px = (ty + 1 + dy) * PMW + (cb + i + 1 + dx);
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(mid + px * PXS + hh * 16 + gk * 8);
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a, acc[q], 0, 0, 0);
+            }
+        }
+        float bv[16];
+#pragma unroll
+        for (int r = 0; r < 16; r++) bv[r] = bb[8 * (r >> 2) + 4 * gk + (r & 3)];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int gq = 2 * wv + q, y = y0 + (gq >> 1), xo = (gq & 1) * 32 + i, x = x0 + xo;
+            if (y >= g.h || xo >= PTW || x >= g.w) continue;
+            const long p = (long(b) * g.h + y) * g.w + x;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                bf16x4 o;
+#pragma unroll
+                for (int c = 0; c < 4; c++) o[c] = to_bf16(gelu(acc[q][4 * j + c] + bv[4 * j + c]));
+                *reinterpret_cast<bf16x4 *>(out + p * NET_C + 8 * j + 4 * gk) = o;
             }
         }
     }
@@ -366,6 +498,13 @@ void net_pack_input(const NetGeo &g, const float *feat, __bf16 *x, hipStream_t s
 static int g_conv_rows = 2;
 void net_set_conv_rows(int rows) { g_conv_rows = rows; }
 int net_conv_rows() { return g_conv_rows; }
+// inference passes run hidden layers two at a time (k_conv_pair); off by
+// default: at 2048 x 256 the pair kernel (60 us) is slower than two
+// single-layer launches (2 x 26 us), its 74 KB of LDS allowing only two
+// workgroups a CU
+static bool g_layer_pairs = false;
+void net_set_layer_pairs(bool on) { g_layer_pairs = on; }
+bool net_layer_pairs() { return g_layer_pairs; }
 
 template <int MODE, int IN, int FTH>
 static void conv_launch(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A,
@@ -405,6 +544,13 @@ void net_conv_first_features(const NetGeo &g, const NetFeatIn &fin, const __bf16
 void net_conv_last_apply(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, const NetApply &ap,
                          hipStream_t st) {
     conv<3>(g, in, wpk, bias, nullptr, nullptr, nullptr, nullptr, NetFeatIn{}, ap, st);
+}
+
+void net_conv_pair(const NetGeo &g, const __bf16 *in, const __bf16 *wa, const float *ba, const __bf16 *wb,
+                   const float *bb, __bf16 *out, hipStream_t st) {
+    const dim3 grid((g.w + PTW - 1) / PTW, (g.h + PTH - 1) / PTH, g.nb);
+    hipLaunchKernelGGL(k_conv_pair, grid, dim3(256), 0, st, g, in, reinterpret_cast<const bf16x8 *>(wa), ba,
+                       reinterpret_cast<const bf16x8 *>(wb), bb, out);
 }
 
 void net_conv_bwd(const NetGeo &g, const __bf16 *dz, const __bf16 *wpk_bwd, const __bf16 *zbelow, __bf16 *dzbelow,

@@ -63,6 +63,8 @@ void net_pack(const float *params, __bf16 *fwd, __bf16 *bwd, hipStream_t st);
 // tile height (2, 4 or 8 rows) of the convolution kernels, process-wide
 void net_set_conv_rows(int rows);
 int net_conv_rows();
+void net_set_layer_pairs(bool on);
+bool net_layer_pairs();
 void net_pack_input(const NetGeo &g, const float *feat, __bf16 *x, hipStream_t st);
 void net_conv_hidden(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A, __bf16 *Z,
                      hipStream_t st);
@@ -70,6 +72,9 @@ void net_conv_last(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const f
                    hipStream_t st);
 void net_conv_bwd(const NetGeo &g, const __bf16 *dz, const __bf16 *wpk_bwd, const __bf16 *zbelow, __bf16 *dzbelow,
                   hipStream_t st);
+// two hidden layers in one pass (inference only: no pre-activations kept)
+void net_conv_pair(const NetGeo &g, const __bf16 *in, const __bf16 *wa, const float *ba, const __bf16 *wb,
+                   const float *bb, __bf16 *out, hipStream_t st);
 void net_wgrad(const NetGeo &g, const __bf16 *x, const __bf16 *dz, float *partial, int nblk, float *grad, int layer,
                hipStream_t st);
 void net_loss(long n, const float *out, const float *target, const float *mask, float inv_msum, __bf16 *dz,
@@ -89,11 +94,11 @@ struct NetScratch {
 // Parameters, optimiser state and activation buffers of one network.
 class GuessNet {
    public:
-    // inference with the caller's buffers: reads only the parameters
-    int infer(const NetGeo &g, const float *feat, float *out, NetScratch &sc, hipStream_t s) const;
     // the solver's guess in one pass: raw features in, guess written to ap.x
     int infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply &ap, NetScratch &sc,
                     hipStream_t s) const;
+    int hidden_step(const NetGeo &g, int l, const __bf16 *in, __bf16 *out, hipStream_t s,
+                    bool pair_ok = true) const;
     int init(int device, hipStream_t st);
     void destroy();
     // activation buffers for up to npx pixels (grown on demand)

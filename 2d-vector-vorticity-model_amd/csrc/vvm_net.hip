@@ -117,21 +117,21 @@ void NetScratch::release() {
     cap = 0;
 }
 
-int GuessNet::infer(const NetGeo &g, const float *feat, float *out, NetScratch &sc, hipStream_t s) const {
-    const long npx = long(g.nb) * g.h * g.w;
-    int rc = sc.reserve(npx);
-    if (rc) return rc;
-    __bf16 *buf[2] = {sc.a0, sc.a1};
-    net_pack_input(g, feat, buf[0], s);
-    for (int l = 0; l < NET_L; l++) {
-        const __bf16 *wl = wf + size_t(l) * NET_PACK;
-        const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
-        if (l == NET_L - 1)
-            net_conv_last(g, buf[l & 1], wl, bl, out, s);
-        else
-            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], nullptr, s);  // no pre-activations
+// Hidden layer l (and l + 1 with it when layer pairs are on, both are
+// hidden layers and `pair_ok`) without keeping pre-activations; returns the
+// number of layers done.
+int GuessNet::hidden_step(const NetGeo &g, int l, const __bf16 *in, __bf16 *out, hipStream_t s,
+                          bool pair_ok) const {
+    const __bf16 *wl = wf + size_t(l) * NET_PACK;
+    const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
+    if (pair_ok && net_layer_pairs() && l + 1 < NET_L - 1) {
+        const __bf16 *wn = wf + size_t(l + 1) * NET_PACK;
+        const float *bn = params + size_t(l + 1) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
+        net_conv_pair(g, in, wl, bl, wn, bn, out, s);
+        return 2;
     }
-    return VVM_OK;
+    net_conv_hidden(g, in, wl, bl, out, nullptr, s);
+    return 1;
 }
 
 int GuessNet::infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply &ap, NetScratch &sc,
@@ -140,15 +140,20 @@ int GuessNet::infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply 
     int rc = sc.reserve(npx);
     if (rc) return rc;
     __bf16 *buf[2] = {sc.a0, sc.a1};
-    for (int l = 0; l < NET_L; l++) {
+    int cur = 1;  // buffer holding the latest activations
+    for (int l = 0; l < NET_L;) {
         const __bf16 *wl = wf + size_t(l) * NET_PACK;
         const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
-        if (l == 0)
+        if (l == 0) {
             net_conv_first_features(g, fin, wl, bl, buf[1], s);
-        else if (l == NET_L - 1)
-            net_conv_last_apply(g, buf[l & 1], wl, bl, ap, s);
-        else
-            net_conv_hidden(g, buf[l & 1], wl, bl, buf[(l + 1) & 1], nullptr, s);
+            l++;
+        } else if (l == NET_L - 1) {
+            net_conv_last_apply(g, buf[cur], wl, bl, ap, s);
+            l++;
+        } else {
+            l += hidden_step(g, l, buf[cur], buf[cur ^ 1], s);
+            cur ^= 1;
+        }
     }
     return VVM_OK;
 }
@@ -159,17 +164,23 @@ int GuessNet::forward(const NetGeo &g, const float *feat, float *out, bool keep,
     if (rc) return rc;
     const hipStream_t st = s ? s : this->st;
     net_pack_input(g, feat, act[0], st);
+    if (!keep) {  // inference: activations alternate between act[0] and act[1]
+        int cur = 0;
+        for (int l = 0; l < NET_L - 1;) {
+            l += hidden_step(g, l, act[cur], act[cur ^ 1], st, l > 0);
+            cur ^= 1;
+        }
+        net_conv_last(g, act[cur], wf + size_t(NET_L - 1) * NET_PACK,
+                      params + size_t(NET_L - 1) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T, out, st);
+        return hipGetLastError() == hipSuccess ? VVM_OK : VVM_ERR_HIP;
+    }
     for (int l = 0; l < NET_L; l++) {
         const __bf16 *wl = wf + size_t(l) * NET_PACK;
         const float *bl = params + size_t(l) * NET_LAYER_PARAMS + NET_C * NET_C * NET_T;
-        const __bf16 *in = keep ? act[l] : act[l & 1];
-        if (l == NET_L - 1) {
-            net_conv_last(g, in, wl, bl, out, st);
-        } else {
-            __bf16 *A = keep ? act[l + 1] : act[(l + 1) & 1];
-            __bf16 *Z = keep ? zpre[l + 1] : zpre[0];
-            net_conv_hidden(g, in, wl, bl, A, Z, st);
-        }
+        if (l == NET_L - 1)
+            net_conv_last(g, act[l], wl, bl, out, st);
+        else
+            net_conv_hidden(g, act[l], wl, bl, act[l + 1], zpre[l + 1], st);
     }
     return hipGetLastError() == hipSuccess ? VVM_OK : VVM_ERR_HIP;
 }
@@ -383,6 +394,12 @@ int vvm_net_adam(vvm_net_t *n, float lr) {
 int vvm_net_set_conv_rows(int rows) {
     if (rows != 2 && rows != 4 && rows != 8) return VVM_ERR_ARG;
     vvmd::net_set_conv_rows(rows);
+    return VVM_OK;
+}
+
+int vvm_net_set_layer_pairs(int on) {
+    if (on != 0 && on != 1) return VVM_ERR_ARG;
+    vvmd::net_set_layer_pairs(on != 0);
     return VVM_OK;
 }
 

@@ -144,6 +144,7 @@ def load_lib(path: str = LIB_PATH):
                                         P(C.c_double)]),
         "vvm_net_adam": (C.c_int, [C.c_void_p, C.c_float]),
         "vvm_net_set_conv_rows": (C.c_int, [C.c_int]),
+        "vvm_net_set_layer_pairs": (C.c_int, [C.c_int]),
         "vvm_dev_collect": (C.c_int, [C.c_void_p, C.c_int]),
         "vvm_dev_collected": (C.c_int, [C.c_void_p]),
         "vvm_net_add_samples": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -606,6 +607,13 @@ class Net:
         rc = load_lib().vvm_net_set_conv_rows(int(rows))
         if rc != 0:
             raise ValueError(f"vvm_net_set_conv_rows({rows}) returned {rc}")
+
+    @staticmethod
+    def set_layer_pairs(on: bool) -> None:
+        """Whether inference runs hidden layers two at a time in one kernel."""
+        rc = load_lib().vvm_net_set_layer_pairs(int(bool(on)))
+        if rc != 0:
+            raise ValueError(f"vvm_net_set_layer_pairs({on}) returned {rc}")
 
     def add_samples(self, dev: "Device") -> None:
         self._check(self.lib.vvm_net_add_samples(self.h, dev.h), "add_samples")

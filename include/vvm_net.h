@@ -58,6 +58,11 @@ int vvm_net_adam(vvm_net_t *n, float lr);
  * of every network in the process: a launch-shape knob like
  * vvm_dev_set_shape, with identical results for every choice. */
 int vvm_net_set_conv_rows(int rows);
+/* 1: inference (vvm_net_forward_host, the solver's learned guess) runs
+ * hidden layers two at a time in one kernel, the activations between them
+ * kept in LDS; 0 (default): one kernel a layer, as training does.
+ * Identical results either way (the same bf16 roundings in the same order). */
+int vvm_net_set_layer_pairs(int on);
 
 /* ---- training data from the model's own runs ---------------------------
  * A timestep handle records one sample per solve (from its third solve on):

@@ -146,6 +146,31 @@ def test_device_forward_matches_oracle(oracle, shape, periodic):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("shape,periodic", [((2, 6, 70), True), ((1, 9, 124), False), ((3, 13, 125), True),
+                                            ((1, 4, 61), False), ((2, 33, 250), False)])
+def test_device_layer_pairs_bit_identical(oracle, shape, periodic):
+    """Two hidden layers in one kernel (the 62 x 4 output tile with the
+    first layer's 64 x 6 region kept in LDS) give exactly the bits of one
+    kernel a layer, at widths and heights that cut the tiles anywhere, and
+    stay within the forward bar of the oracle."""
+    nb, h, w = shape
+    p = random_params(40 + h)
+    inp, _, _ = sample(41 + w, nb, h, w)
+    print(f"net layer pairs {shape} periodic={periodic}", flush=True)
+    try:
+        with V.Net() as net:
+            net.set_params(p)
+            V.Net.set_layer_pairs(False)
+            single = net.forward(inp, periodic)
+            V.Net.set_layer_pairs(True)
+            paired = net.forward(inp, periodic)
+    finally:
+        V.Net.set_layer_pairs(False)
+    np.testing.assert_array_equal(paired, single)
+    assert rel(paired, oracle.net_forward(p, inp, periodic)) <= FWD_TOL
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("shape,periodic", [((2, 8, 70), False), ((1, 20, 64), True)])
 def test_device_gradient_matches_oracle(oracle, shape, periodic):
     """The device's backward kernels (data and weight gradients of every
