@@ -22,6 +22,7 @@
 // runs over pixels (K) with the input tile and the output gradient staged
 // channel-major in LDS; partial sums per workgroup are reduced by
 // k_wgrad_reduce.
+#include <algorithm>
 #include <cstdint>
 
 #include "net.h"
@@ -86,11 +87,14 @@ __device__ inline long halo_src(const NetGeo &g, int b, int y0, int x0, int ry, 
 // input is the three raw FP32 feature planes (NetFeatIn), scaled and
 // rounded to bf16 while staged; the other 29 channels are zero.
 //
-// A workgroup of 4 wavefronts computes a 64 x FTH pixel tile from the
-// (66 x FTH+2)-pixel haloed input staged once in LDS; the tile holds 2 FTH
-// MFMA output blocks of 32 pixels x 32 channels, FTH / 2 per wavefront, and
-// a wavefront applies each weight fragment it loads to all of its blocks.
-// Taller tiles re-read less halo; shorter ones fit more workgroups per CU.
+// A workgroup of 4 wavefronts computes 64 x FTH pixel tiles from the
+// (66 x FTH+2)-pixel haloed input staged in LDS; a tile holds 2 FTH MFMA
+// output blocks of 32 channels x 32 pixels, FTH / 2 per wavefront, and a
+// wavefront applies each weight fragment to all of its blocks.  The
+// workgroups stay resident and walk the tiles with a stride of the grid:
+// a lane loads its 18 weight fragments once, and the next tile's input is
+// fetched into registers while the current tile's products run, so neither
+// the weights' nor the input's memory latency sits between two tiles.
 template <int MODE, int IN = 0, int FTH = 4>
 __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict__ in,
                                               const bf16x8 *__restrict__ wpk, const float *__restrict__ bias,
@@ -98,98 +102,133 @@ __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict
                                               float *__restrict__ outF, const __bf16 *__restrict__ zbelow,
                                               NetFeatIn fin, NetApply ap) {
     constexpr int FHH = FTH + 2, BPW = FTH / 2;  // haloed rows, output blocks per wavefront
+    constexpr int NCH = FHH * FHW * 4;           // 16-byte chunks of a haloed tile (IN 0)
+    constexpr int NPX = FHH * FHW;               // its pixels (IN 1: one thread a pixel)
+    constexpr int NPF = IN == 0 ? (NCH + 255) / 256 : (NPX + 255) / 256;
     __shared__ __attribute__((aligned(16))) __bf16 tile[FHH * FHW * PXS];
-    const int b = blockIdx.z, y0 = blockIdx.y * FTH, x0 = blockIdx.x * FTW;
-    if (IN == 0) {
-        // stage the haloed tile: 16-byte chunks, 4 per pixel
-        for (int c = threadIdx.x; c < FHH * FHW * 4; c += blockDim.x) {
-            const int px = c >> 2, part = c & 3;
-            const int ry = px / FHW, rx = px % FHW;
-            const long src = halo_src(g, b, y0, x0, ry, rx);
-            bf16x8 v = {};
-            if (src >= 0) v = *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8);
-            *reinterpret_cast<bf16x8 *>(tile + px * PXS + part * 8) = v;
-        }
-    } else {
-        const long hw = long(g.h) * g.w;
-        double sc[3];
-#pragma unroll
-        for (int c = 0; c < 3; c++) sc[c] = inv_rms(fin.sumsq[c], fin.npts);
-        for (int px = threadIdx.x; px < FHH * FHW; px += blockDim.x) {
-            const long src = halo_src(g, b, y0, x0, px / FHW, px % FHW);
-            bf16x8 lo = {}, z = {};
-            if (src >= 0) {
-                const long q = src - long(b) * hw;
-#pragma unroll
-                for (int c = 0; c < 3; c++)
-                    lo[c] = to_bf16(float(double(fin.feat[(long(b) * 3 + c) * hw + q]) * sc[c]));
-            }
-            bf16x8 *dst = reinterpret_cast<bf16x8 *>(tile + px * PXS);
-            dst[0] = lo;
-            dst[1] = z;
-            dst[2] = z;
-            dst[3] = z;
-        }
-    }
-    __syncthreads();
+    const int gx = (g.w + FTW - 1) / FTW, gy = (g.h + FTH - 1) / FTH;
+    const int ntile = gx * gy * g.nb;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lane & 31, gk = lane >> 5;
-    f32x16 acc[BPW] = {};  // block gq = BPW wv + q: row gq / 2, columns 32 (gq % 2) ..
+    // IN 1: channels 3..31 of the input are zero, so the steps over input
+    // channels 16..31 (odd s) add nothing and are skipped
+    bf16x8 wr[18];
 #pragma unroll
-    for (int s = 0; s < 18; s++) {
-        const int t = s >> 1, hh = s & 1;
-        const int dy = t / 3 - 1, dx = t % 3 - 1;
-        const bf16x8 w = wpk[s * 64 + lane];
-#pragma unroll
-        for (int q = 0; q < BPW; q++) {
-            const int gq = BPW * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
-            const int px = (ty + 1 + dy) * FHW + (cb + i + 1 + dx);
-            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
-            acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a, acc[q], 0, 0, 0);
-        }
-    }
-    // weights as the A operand, pixels as B: D[m][n] with n = lane % 32 the
-    // pixel and m = 8 (r / 4) + 4 (lane / 32) + r % 4 the output channel, so
-    // a lane holds four runs of 4 consecutive channels of one pixel
+    for (int s = 0; s < 18; s++)
+        if (IN == 0 || (s & 1) == 0) wr[s] = wpk[s * 64 + lane];
     float bv[16];
     if (MODE == 0) {
 #pragma unroll
         for (int r = 0; r < 16; r++) bv[r] = bias[8 * (r >> 2) + 4 * gk + (r & 3)];
     }
+    double sc[3];
+    if (IN == 1) {
 #pragma unroll
-    for (int q = 0; q < BPW; q++) {
-        const int gq = BPW * wv + q, y = y0 + (gq >> 1), x = x0 + (gq & 1) * 32 + i;
-        if (y >= g.h || x >= g.w) continue;
-        const long p = (long(b) * g.h + y) * g.w + x;
-        if (MODE == 0 || MODE == 2) {
+        for (int c = 0; c < 3; c++) sc[c] = inv_rms(fin.sumsq[c], fin.npts);
+    }
+    const long hw = long(g.h) * g.w;
+    bf16x8 pf[IN == 0 ? NPF : 1];     // IN 0: prefetched chunks
+    float pff[IN == 1 ? NPF : 1][3];  // IN 1: prefetched feature values
+    auto fetch = [&](int t) {
+        const int b = t / (gx * gy), y0 = (t / gx) % gy * FTH, x0 = t % gx * FTW;
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const long e = p * NET_C + 8 * j + 4 * gk;  // channels 8 j + 4 gk .. + 3
-                bf16x4 o;
-                if (MODE == 0) {
-                    bf16x4 zq;
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const float z = acc[q][4 * j + c] + bv[4 * j + c];
-                        zq[c] = to_bf16(z);
-                        o[c] = to_bf16(gelu(z));
-                    }
-                    if (outZ) *reinterpret_cast<bf16x4 *>(outZ + e) = zq;  // kept for training only
-                } else {
-                    const bf16x4 zb = *reinterpret_cast<const bf16x4 *>(zbelow + e);
-#pragma unroll
-                    for (int c = 0; c < 4; c++) o[c] = to_bf16(acc[q][4 * j + c] * gelu_d(from_bf16(zb[c])));
-                }
-                *reinterpret_cast<bf16x4 *>(outA + e) = o;
-            }
-        } else if (gk == 0) {  // channel 0 is m = 0: register 0 of lanes 0..31
-            if (MODE == 1) {
-                outF[p] = acc[q][0] + bias[0];
+        for (int k = 0; k < NPF; k++) {
+            if (IN == 0) {
+                const int c = threadIdx.x + 256 * k, px = c >> 2, part = c & 3;
+                const long src = c < NCH ? halo_src(g, b, y0, x0, px / FHW, px % FHW) : -1;
+                pf[k] = src >= 0 ? *reinterpret_cast<const bf16x8 *>(in + src * NET_C + part * 8) : bf16x8{};
             } else {
+                const int px = threadIdx.x + 256 * k;
+                const long src = px < NPX ? halo_src(g, b, y0, x0, px / FHW, px % FHW) : -1;
+                const long q = src - long(b) * hw;
+#pragma unroll
+                for (int c = 0; c < 3; c++) pff[k][c] = src >= 0 ? fin.feat[(long(b) * 3 + c) * hw + q] : 0.f;
+            }
+        }
+    };
+    auto put = [&]() {
+#pragma unroll
+        for (int k = 0; k < NPF; k++) {
+            if (IN == 0) {
+                const int c = threadIdx.x + 256 * k;
+                if (c < NCH) *reinterpret_cast<bf16x8 *>(tile + (c >> 2) * PXS + (c & 3) * 8) = pf[k];
+            } else {
+                const int px = threadIdx.x + 256 * k;
+                if (px < NPX) {
+                    bf16x8 lo = {}, z = {};
+#pragma unroll
+                    for (int c = 0; c < 3; c++) lo[c] = to_bf16(float(double(pff[k][c]) * sc[c]));
+                    bf16x8 *dst = reinterpret_cast<bf16x8 *>(tile + px * PXS);
+                    dst[0] = lo;
+                    dst[1] = z;
+                    dst[2] = z;
+                    dst[3] = z;
+                }
+            }
+        }
+    };
+    int t = blockIdx.x;
+    if (t < ntile) fetch(t);
+    for (; t < ntile; t += gridDim.x) {
+        const int b = t / (gx * gy), y0 = (t / gx) % gy * FTH, x0 = t % gx * FTW;
+        __syncthreads();  // the previous tile's reads of the LDS tile are done
+        put();
+        __syncthreads();
+        if (t + int(gridDim.x) < ntile) fetch(t + gridDim.x);
+        f32x16 acc[BPW] = {};  // block gq = BPW wv + q: row gq / 2, columns 32 (gq % 2) ..
+#pragma unroll
+        for (int s = 0; s < 18; s++) {
+            if (IN == 1 && (s & 1)) continue;
+            const int tap = s >> 1, hh = s & 1;
+            const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+#pragma unroll
+            for (int q = 0; q < BPW; q++) {
+                const int gq = BPW * wv + q, ty = gq >> 1, cb = (gq & 1) * 32;
+                const int px = (ty + 1 + dy) * FHW + (cb + i + 1 + dx);
+                const bf16x8 a = *reinterpret_cast<const bf16x8 *>(tile + px * PXS + hh * 16 + gk * 8);
+                acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wr[s], a, acc[q], 0, 0, 0);
+            }
+        }
+        // weights as the A operand, pixels as B: D[m][n] with n = lane % 32
+        // the pixel and m = 8 (r / 4) + 4 (lane / 32) + r % 4 the output
+        // channel, so a lane holds four runs of 4 consecutive channels of one
+        // pixel
+#pragma unroll
+        for (int q = 0; q < BPW; q++) {
+            const int gq = BPW * wv + q, y = y0 + (gq >> 1), x = x0 + (gq & 1) * 32 + i;
+            if (y >= g.h || x >= g.w) continue;
+            const long p = (long(b) * g.h + y) * g.w + x;
+            if (MODE == 0 || MODE == 2) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const long e = p * NET_C + 8 * j + 4 * gk;  // channels 8 j + 4 gk .. + 3
+                    bf16x4 o;
+                    if (MODE == 0) {
+                        bf16x4 zq;
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            const float z = acc[q][4 * j + c] + bv[4 * j + c];
+                            zq[c] = to_bf16(z);
+                            o[c] = to_bf16(gelu(z));
+                        }
+                        if (outZ) *reinterpret_cast<bf16x4 *>(outZ + e) = zq;  // kept for training only
+                    } else {
+                        const bf16x4 zb = *reinterpret_cast<const bf16x4 *>(zbelow + e);
+#pragma unroll
+                        for (int c = 0; c < 4; c++)
+                            o[c] = to_bf16(acc[q][4 * j + c] * gelu_d(from_bf16(zb[c])));
+                    }
+                    *reinterpret_cast<bf16x4 *>(outA + e) = o;
+                }
+            } else if (gk == 0) {  // channel 0 is m = 0: register 0 of lanes 0..31
+                if (MODE == 1) {
+                    outF[p] = acc[q][0] + bias[0];
+                } else {
 #pragma clang fp contract(off)
-                const float o = acc[q][0] + bias[0];
-                const size_t a = size_t(y + 1) * ap.N + x;
-                ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+                    const float o = acc[q][0] + bias[0];
+                    const size_t a = size_t(y + 1) * ap.N + x;
+                    ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+                }
             }
         }
     }
@@ -510,7 +549,18 @@ template <int MODE, int IN, int FTH>
 static void conv_launch(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A,
                         __bf16 *Z, float *F, const __bf16 *zb, const NetFeatIn &fin, const NetApply &ap,
                         hipStream_t st) {
-    const dim3 grid((g.w + FTW - 1) / FTW, (g.h + FTH - 1) / FTH, g.nb);
+    // resident workgroups: as many as fit on the device at once, at most one a tile
+    static int per_cu = 0, ncu = 0;
+    if (!per_cu) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_conv<MODE, IN, FTH>),
+                                                     256, 0);
+        if (per_cu < 1) per_cu = 1;
+    }
+    const long ntile = long((g.w + FTW - 1) / FTW) * ((g.h + FTH - 1) / FTH) * g.nb;
+    const dim3 grid(unsigned(std::min<long>(ntile, long(per_cu) * ncu)));
     hipLaunchKernelGGL((k_conv<MODE, IN, FTH>), grid, dim3(256), 0, st, g, in,
                        reinterpret_cast<const bf16x8 *>(wpk), bias, A, Z, F, zb, fin, ap);
 }

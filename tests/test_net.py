@@ -246,3 +246,7 @@ def test_learned_guess_gain():
     assert ts.loss_last < ts.loss_first
     assert e1 < 0.9 * e0 and e1 < 0.9
     assert s_le.not_converged == 0 and s_le.rel_residual_last <= 1e-10
+    # the solver's own inference path (raw features scaled while staged, the
+    # guess written by the last layer) must give a guess at least as good as
+    # the extrapolation: a wrong guess costs iterations in every solve
+    assert s_le.solve_iters_total <= 1.1 * s_ex.solve_iters_total, (s_le.solve_iters_total, s_ex.solve_iters_total)
