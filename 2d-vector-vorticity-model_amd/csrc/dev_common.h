@@ -112,17 +112,16 @@ __device__ inline void block_partials(double (&v)[NV], const RedSlot &rs) {
 }
 
 // Scalars of the preconditioned CG (doubles in its scalar block).
-// P_RZ / P_RZ2: r.z of odd / even CG iterations (two slots, so that the
+// P_RZ / P_RZ2: r.z of odd / even CG iterations, P_ALPHA / P_ALPHA2 the
+// step lengths of the single-reduction form (two slots each, so that the
 // kernel forming the new value never overwrites the one its other
-// workgroups still read).
-enum { P_RZ = 0, P_PQ, P_ALPHA, P_BETA, P_RNORM2, P_BNORM2, P_RNORM2_0, P_RZ2, NPSCAL = 8 };
+// workgroups still read).  Slot 1 is unused.
+enum { P_RZ = 0, P_SPARE, P_ALPHA, P_ALPHA2, P_RNORM2, P_BNORM2, P_RNORM2_0, P_RZ2, NPSCAL = 8 };
 
 // What k_finalize does with the totals.
 enum RedMode {
     RED_STORE = 0,   // sc[dst+j] = tot[j] (and sc[dst2+j] when dst2 >= 0)
     RED_NEGVAL,      // NegativeValueProcess ratios of 3 fields into sc[dst+q]
-    RED_ALPHA,       // CG: sc[P_PQ] = p.q, sc[P_ALPHA] = sc[P_RZ] / p.q
-    RED_BETA,        // CG: sc[P_BETA] = rz / sc[P_RZ], sc[P_RZ] = rz
     RED_RZ_INIT      // CG: sc[P_RZ] = rz
 };
 
@@ -156,14 +155,6 @@ __global__ void __launch_bounds__(256) k_finalize(RedSlot rs, int nblk, double *
                 const double P = acc[2 * q], N = acc[2 * q + 1];
                 sc[dst + q] = (P == 0. || fabs(N) > P) ? -1. : 1. - fabs(N / P);
             }
-            break;
-        case RED_ALPHA:
-            sc[P_PQ] = acc[0];
-            sc[P_ALPHA] = sc[P_RZ] / acc[0];
-            break;
-        case RED_BETA:
-            sc[P_BETA] = acc[0] / sc[P_RZ];
-            sc[P_RZ] = acc[0];
             break;
         case RED_RZ_INIT:
             sc[P_RZ] = acc[0];

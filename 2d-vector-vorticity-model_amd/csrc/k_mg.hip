@@ -24,8 +24,10 @@ namespace {
 
 constexpr int CX = 32, CY = 4;  // coarse points per block of the down kernel
 constexpr int TX = 64, TY = 4;  // fine points per block of the up kernel
-constexpr int DW = 2 * CX + 5, DH = 2 * CY + 5;
-constexpr int UW = TX + 4, UH = TY + 4;
+// the level-0 kernels of the single-reduction CG carry more halo (and more
+// vectors), so they take taller tiles
+constexpr int CYG = 8, TYG = 8;
+constexpr int DW = 2 * CX + 5;
 
 __host__ __device__ inline int wrapi(int i, int N) {
     i %= N;
@@ -46,12 +48,31 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
                                                  const T *__restrict__ cx, const T *__restrict__ cu,
                                                  const T *__restrict__ dinv, T omega, int Nc, int Mc,
                                                  T *__restrict__ bc, CGFuse<T> cg) {
+    constexpr int CYk = CG ? CYG : CY, DH = 2 * CYk + 5;
     __shared__ T Bs[DH * DW], X1[DH * DW], X2[DH * DW];
     __shared__ T Rx[DH], Ru[DH], Rd[DH];  // cx, cu, dinv of the tile's rows
-    T alpha = T(0);
-    if (CG) alpha = T(cg.sc[cg.rz] / block_total(cg.rs_pq, cg.npq));
-    const int ic0 = blockIdx.x * CX, jc0 = 1 + blockIdx.y * CY;
-    const int ic1 = min(ic0 + CX - 1, Nc - 1), jc1 = min(jc0 + CY - 1, Mc);
+    T alpha = T(0), beta = T(0);
+    if (CG) {
+        // gamma = r.z, delta = w.z of the last V-cycle; beta = gamma /
+        // gamma_old, alpha = gamma / (delta - beta gamma / alpha_old)
+        const double gam = block_total(cg.rs_gd, cg.ngd);
+        const double del = block_total(RedSlot{cg.rs_gd.partial + cg.rs_gd.max_blocks, cg.rs_gd.max_blocks}, cg.ngd);
+        double a, bt = 0.;
+        if (cg.first) {
+            a = gam / del;
+        } else {
+            bt = gam / cg.sc[cg.g_old];
+            a = gam / (del - bt * gam / cg.sc[cg.a_old]);
+        }
+        alpha = T(a);
+        beta = T(bt);
+        if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+            cg.sc[cg.g_new] = gam;
+            cg.sc[cg.a_new] = a;
+        }
+    }
+    const int ic0 = blockIdx.x * CX, jc0 = 1 + blockIdx.y * CYk;
+    const int ic1 = min(ic0 + CX - 1, Nc - 1), jc1 = min(jc0 + CYk - 1, Mc);
     const int fi0 = 2 * ic0 - 3, fj0 = 2 * jc0 - 3;
     const int W = 2 * (ic1 - ic0) + 7, H = 2 * (jc1 - jc0) + 7;
     const int n = W * H;
@@ -69,7 +90,17 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
         T bv = T(0);
         if (in) {
             const size_t a = size_t(j) * N + i;
-            bv = CG ? cg.rin[a] - alpha * cg.q[a] : b[a];
+            if (CG) {
+                // q = w + beta q_old (= L p), r = r_old - alpha q; q is
+                // stored by the workgroup that owns the point
+                const T qn = cg.first ? cg.w[a] : cg.w[a] + beta * cg.qin[a];
+                bv = cg.rin[a] - alpha * qn;
+                const int oi = i - 2 * ic0, oj = j - (2 * jc0 - 1);
+                const int jtop = (jc1 == Mc) ? 2 * jc1 + 1 : 2 * jc1;
+                if (fi0 + ti == i && oi >= 0 && oi < 2 * (ic1 - ic0) + 2 && oj >= 0 && j <= jtop) cg.qout[a] = qn;
+            } else {
+                bv = b[a];
+            }
         }
         Bs[tj * DW + ti] = bv;
         X1[tj * DW + ti] = in ? omega * dinv[j] * bv : T(0);
@@ -97,14 +128,18 @@ __global__ void __launch_bounds__(256) k_mg_down(int N, int M, const T *__restri
         if (j >= 1 && j <= M) {
             const size_t a = size_t(j) * N + i;
             const int s = (j - fj0) * DW + (i - fi0);
-            x2out[a] = X2[s];
             if (CG) {  // the CG update on the points this workgroup owns
+                // p = z + beta p_old; z (in x2out's buffer) is read before
+                // the smoothed value below replaces it
+                const T pn = cg.first ? x2out[a] : x2out[a] + beta * cg.p[a];
+                cg.p[a] = pn;
+                cg.x[a] = cg.x[a] + alpha * pn;
                 const T rn = Bs[s];
                 cg.rout[a] = rn;
-                cg.x[a] = cg.x[a] + alpha * cg.p[a];
                 const double wr = double(cg.wD[j]) * double(rn);
                 part[0] += wr * wr;
             }
+            x2out[a] = X2[s];
         }
     }
     if (CG) {
@@ -146,17 +181,23 @@ __device__ inline T prolong(const T *__restrict__ xc, int Nc, int i, int j) {
     return T(0.25) * (r0[ic] + r0[ic1] + r1[ic] + r1[ic1]);
 }
 
-template <typename T>
+// CGW (level 0 of the single-reduction CG): the tile carries a 3-point
+// halo, the second sweep runs on the owned points and a 1-point ring around
+// them (kept in LDS), and the kernel also writes w = L z and the partials of
+// r.z and w.z (rows 0 and 1 of rs).  The ring values are the ones the
+// neighbouring workgroups compute for their own points, by the same code.
+template <typename T, bool CGW>
 __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict__ b, const T *__restrict__ x2,
                                                T *__restrict__ xout, const T *__restrict__ cx,
                                                const T *__restrict__ cu, const T *__restrict__ dinv, T omega, int Nc,
-                                               const T *__restrict__ xc, RedSlot rs, int dot) {
-    __shared__ T Xs[UH * UW], X3[UH * UW], Bs[UH * UW];
-    __shared__ T Rx[UH], Ru[UH], Rd[UH];  // cx, cu, dinv of the tile's rows
-    const int i0 = blockIdx.x * TX, j0 = 1 + blockIdx.y * TY;
-    const int fi0 = i0 - 2, fj0 = j0 - 2;
-    const int n = UW * UH;
-    if (threadIdx.x < UH) {  // loaded with the tile, so later steps read LDS only
+                                               const T *__restrict__ xc, RedSlot rs, int dot, T *__restrict__ wout) {
+    constexpr int TYk = CGW ? TYG : TY, HL = CGW ? 3 : 2, W = TX + 2 * HL, H = TYk + 2 * HL;
+    __shared__ T Xs[H * W], X3[H * W], Bs[H * W];
+    __shared__ T Rx[H], Ru[H], Rd[H];  // cx, cu, dinv of the tile's rows
+    const int i0 = blockIdx.x * TX, j0 = 1 + blockIdx.y * TYk;
+    const int fi0 = i0 - HL, fj0 = j0 - HL;
+    const int n = W * H;
+    if (threadIdx.x < H) {  // loaded with the tile, so later steps read LDS only
         const int j = fj0 + threadIdx.x;
         const bool in = (j >= 1 && j <= M);
         Rx[threadIdx.x] = in ? cx[j] : T(0);
@@ -164,7 +205,7 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
         Rd[threadIdx.x] = in ? dinv[j] : T(0);
     }
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int tj = t / UW, ti = t % UW;
+        const int tj = t / W, ti = t % W;
         const int j = fj0 + tj, i = wrapi(fi0 + ti, N);
         const bool in = (j >= 1 && j <= M);
         T v = T(0), bv = T(0);
@@ -179,24 +220,50 @@ __global__ void __launch_bounds__(256) k_mg_up(int N, int M, const T *__restrict
     }
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const int tj = t / UW, ti = t % UW;
+        const int tj = t / W, ti = t % W;
         const int j = fj0 + tj;
         T v = T(0);
-        if (ti >= 1 && ti <= UW - 2 && tj >= 1 && tj <= UH - 2 && j >= 1 && j <= M)
-            v = Xs[t] + omega * Rd[tj] * (Bs[t] - lap(Xs, UW, t, Rx[tj], Ru[tj], Ru[tj - 1]));
+        if (ti >= 1 && ti <= W - 2 && tj >= 1 && tj <= H - 2 && j >= 1 && j <= M)
+            v = Xs[t] + omega * Rd[tj] * (Bs[t] - lap(Xs, W, t, Rx[tj], Ru[tj], Ru[tj - 1]));
         X3[t] = v;
     }
     __syncthreads();
-    double part[1] = {0.};
-    for (int t = threadIdx.x; t < TX * TY; t += blockDim.x) {
+    double part[2] = {0., 0.};
+    if (!CGW) {
+        for (int t = threadIdx.x; t < TX * TYk; t += blockDim.x) {
+            const int i = i0 + t % TX, j = j0 + t / TX;
+            if (i >= N || j > M) continue;
+            const int s = (j - fj0) * W + (i - fi0), tj = j - fj0;
+            const T z = X3[s] + omega * Rd[tj] * (Bs[s] - lap(X3, W, s, Rx[tj], Ru[tj], Ru[tj - 1]));
+            xout[size_t(j) * N + i] = z;
+            part[0] += double(Bs[s]) * double(z);
+        }
+        double p1[1] = {part[0]};
+        if (dot) block_partials<1>(p1, rs);
+        return;
+    }
+    // second sweep on the owned points and their 1-point ring, into Xs
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+        const int tj = t / W, ti = t % W;
+        const int j = fj0 + tj;
+        if (ti >= 2 && ti <= W - 3 && tj >= 2 && tj <= H - 3)
+            Xs[t] = (j >= 1 && j <= M) ? X3[t] + omega * Rd[tj] * (Bs[t] - lap(X3, W, t, Rx[tj], Ru[tj], Ru[tj - 1]))
+                                       : T(0);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < TX * TYk; t += blockDim.x) {
         const int i = i0 + t % TX, j = j0 + t / TX;
         if (i >= N || j > M) continue;
-        const int s = (j - fj0) * UW + (i - fi0), tj = j - fj0;
-        const T z = X3[s] + omega * Rd[tj] * (Bs[s] - lap(X3, UW, s, Rx[tj], Ru[tj], Ru[tj - 1]));
-        xout[size_t(j) * N + i] = z;
+        const int s = (j - fj0) * W + (i - fi0), tj = j - fj0;
+        const T z = Xs[s];
+        const T w = lap(Xs, W, s, Rx[tj], Ru[tj], Ru[tj - 1]);
+        const size_t a = size_t(j) * N + i;
+        xout[a] = z;
+        wout[a] = w;
         part[0] += double(Bs[s]) * double(z);
+        part[1] += double(w) * double(z);
     }
-    if (dot) block_partials<1>(part, rs);
+    block_partials<2>(part, rs);
 }
 
 // All levels from Lc down, in one workgroup with every array in LDS.
@@ -425,12 +492,21 @@ int mg_plan(MGHier<T> &H, size_t lds_budget) {
 }
 
 template <typename T>
-int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStream_t st) {
-    if (H.coarse_from < 1 || H.nlev < 2) return 0;
+bool mg_cg_fusable(const MGHier<T> &H, int max_blocks) {
+    if (H.coarse_from < 1 || H.nlev < 2) return false;
     const MGLevel<T> &F = H.lev[0];
     const MGLevel<T> &C = H.lev[1];
-    const dim3 g((C.N + CX - 1) / CX, (C.M + CY - 1) / CY);
-    if (int(g.x * g.y) > max_blocks) return 0;
+    const long nd = long((C.N + CX - 1) / CX) * ((C.M + CYG - 1) / CYG);
+    const long nu = long((F.N + TX - 1) / TX) * ((F.M + TYG - 1) / TYG);
+    return nd <= max_blocks && nu <= max_blocks;
+}
+
+template <typename T>
+int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStream_t st) {
+    if (!mg_cg_fusable(H, max_blocks)) return 0;
+    const MGLevel<T> &F = H.lev[0];
+    const MGLevel<T> &C = H.lev[1];
+    const dim3 g((C.N + CX - 1) / CX, (C.M + CYG - 1) / CYG);
     hipLaunchKernelGGL((k_mg_down<T, true>), g, dim3(256), 0, st, F.N, F.M, f.rin, zA, F.cx, F.cu, F.dinv,
                        H.omega, C.N, C.M, C.b, f);
     return int(g.x * g.y);
@@ -438,7 +514,7 @@ int mg_down0_cg(MGHier<T> &H, T *zA, const CGFuse<T> &f, int max_blocks, hipStre
 
 template <typename T>
 int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
-              bool skip_down0) {
+              bool skip_down0, T *w) {
     int nrz = 0;
     const int Lc = H.coarse_from, L = H.nlev;
     // level-l arrays: level 0 uses the CG's r (b) and z buffers
@@ -465,13 +541,19 @@ int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doubl
         const T *xc = (l + 1 < L) ? H.lev[l + 1].x : nullptr;
         T *x2 = (l == 0) ? zA : F.x;
         T *xo = (l == 0) ? zB : F.t;
-        const dim3 g((F.N + TX - 1) / TX, (F.M + TY - 1) / TY);
-        hipLaunchKernelGGL(k_mg_up<T>, g, dim3(256), 0, st, F.N, F.M, bptr(l), x2, xo, F.cx, F.cu, F.dinv, H.omega,
-                           (l + 1 < L) ? H.lev[l + 1].N : 1, xc, rs, int(l == 0));
+        const bool cgw = (l == 0 && w);
+        const dim3 g((F.N + TX - 1) / TX, (F.M + (cgw ? TYG : TY) - 1) / (cgw ? TYG : TY));
+        const int Nc = (l + 1 < L) ? H.lev[l + 1].N : 1;
+        if (cgw)
+            hipLaunchKernelGGL((k_mg_up<T, true>), g, dim3(256), 0, st, F.N, F.M, bptr(l), x2, xo, F.cx, F.cu,
+                               F.dinv, H.omega, Nc, xc, rs, 1, w);
+        else
+            hipLaunchKernelGGL((k_mg_up<T, false>), g, dim3(256), 0, st, F.N, F.M, bptr(l), x2, xo, F.cx, F.cu,
+                               F.dinv, H.omega, Nc, xc, rs, int(l == 0), static_cast<T *>(nullptr));
         if (l == 0) {
             std::swap(zA, zB);
             nrz = int(g.x * g.y);
-            if (init) finalize<1>(rs, nrz, sc, RED_RZ_INIT, 0, st);
+            if (init && !w) finalize<1>(rs, nrz, sc, RED_RZ_INIT, 0, st);
         } else {
             std::swap(F.x, F.t);
         }
@@ -490,9 +572,11 @@ int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, doubl
 template int mg_plan<double>(MGHier<double> &, size_t);
 template int mg_plan<float>(MGHier<float> &, size_t);
 template int mg_vcycle<double>(MGHier<double> &, const double *, double *&, double *&, const RedSlot &, double *,
-                               bool, hipStream_t, bool);
+                               bool, hipStream_t, bool, double *);
 template int mg_vcycle<float>(MGHier<float> &, const float *, float *&, float *&, const RedSlot &, double *, bool,
-                              hipStream_t, bool);
+                              hipStream_t, bool, float *);
+template bool mg_cg_fusable<double>(const MGHier<double> &, int);
+template bool mg_cg_fusable<float>(const MGHier<float> &, int);
 template int mg_down0_cg<double>(MGHier<double> &, double *, const CGFuse<double> &, int, hipStream_t);
 template int mg_down0_cg<float>(MGHier<float> &, float *, const CGFuse<float> &, int, hipStream_t);
 

@@ -485,17 +485,26 @@ void free_hier(MGHier<T> &H) {
 // after the test, since a speculative V-cycle after the last iteration is
 // wasted work; a wrong prediction costs one drained queue instead.  Returns
 // the iteration count; *rn2 gets the last residual norm^2 seen.
+//
+// On a hierarchy with a tiled level 0 (every grid but the smallest) the
+// iteration is the single-reduction form: the V-cycle's level-0 up kernel
+// leaves r.z and w.z (w = L z), and the level-0 down kernel of the next
+// V-cycle forms alpha and beta from them and applies the whole update, so an
+// iteration is one V-cycle and nothing else.  Otherwise (a hierarchy that is
+// only the coarse kernel) it is standard PCG with its own vector kernels.
 template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
              double *hs_dev, const double *hpart, double *hpart_dev, hipEvent_t ev, hipStream_t st,
              double rn_start, double *rn2, bool *conv) {
+    const bool fused = hpart_dev && mg_cg_fusable(P.mg, rs.max_blocks);
     const int nb = cg_blocks(P.N, P.M);
     const dim3 b(LX, LY);
     // p.q partials in the last row of the reduction slots, which the update
     // kernel reads while it writes its own partials to row 0
     const RedSlot rs_pq{rs.partial + size_t(7) * rs.max_blocks, rs.max_blocks};
-    mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st);  // r.z into sc[P_RZ]
-    int n_rz = 0;  // r.z partials of the last V-cycle, summed by the next pq
+    // fused: r.z and w.z partials; otherwise r.z into sc[P_RZ]
+    int n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st, false, fused ? P.w : nullptr);
+    if (!fused) n_rz = 0;  // the first pq kernel takes beta = 0 and needs no partials
     int it = 0;
     double best = 1e300;
     int stall = 0;
@@ -504,17 +513,18 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
     while (it < max_iter) {
         it++;
         const int rz_old = (it & 1) ? P_RZ2 : P_RZ, rz_now = (it & 1) ? P_RZ : P_RZ2;  // it = 1: P_RZ
-        hipLaunchKernelGGL(k_pcg_pq<T>, dim3(nb), b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
-                           P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1), rs, n_rz, rz_old, rz_now);
-        std::swap(P.pA, P.pB);
-        // the update inside the level-0 down kernel when the hierarchy has
-        // one (its workgroups leave ||w r||^2 partials in host memory), else
-        // its own kernel and a reduction into the host word hs
-        const CGFuse<T> fz{P.x, P.r, P.rB, P.pA, P.q, P.wD, P.sc, rz_now, rs_pq, nb, hpart_dev};
-        const int nfused = hpart_dev ? mg_down0_cg(P.mg, P.zA, fz, rs.max_blocks, st) : 0;
-        if (nfused) {
+        int nfused = 0;
+        if (fused) {
+            const int a_old = (it & 1) ? P_ALPHA2 : P_ALPHA, a_now = (it & 1) ? P_ALPHA : P_ALPHA2;
+            const CGFuse<T> fz{P.x,  P.r,   P.rB,   P.pA,  P.q,         P.qB, P.w, P.wD, P.sc, rz_old,
+                               rz_now, a_old, a_now, int(it == 1), rs, n_rz, hpart_dev};
+            nfused = mg_down0_cg(P.mg, P.zA, fz, rs.max_blocks, st);
             std::swap(P.r, P.rB);
+            std::swap(P.q, P.qB);
         } else {
+            hipLaunchKernelGGL(k_pcg_pq<T>, dim3(nb), b, 0, st, P.N, P.M, P.zA, P.pA, P.pB, P.q, P.mg.lev[0].cx,
+                               P.mg.lev[0].cu, P.sc, rs_pq, int(it == 1), rs, n_rz, rz_old, rz_now);
+            std::swap(P.pA, P.pB);
             hipLaunchKernelGGL(k_pcg_update<T>, dim3(nb), b, 0, st, P.N, P.M, P.x, P.r, P.pA, P.q, P.wD, P.sc,
                                rz_now, rs_pq, nb, rs);
             finalize<1>(rs, nb, P.sc, RED_STORE, P_RNORM2, st, -1, hs_dev);
@@ -527,10 +537,11 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             const double q = (prev2 > 0.) ? std::min(prev / prev2, 1.) : 1e-2;
             if (prev * q <= target) ahead = false;
         }
-        if (ahead) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
+        T *wz = fused ? P.w : nullptr;
+        if (ahead) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, fused, wz);
         hipEventSynchronize(ev);
         double rn = 0.;
-        if (nfused) {
+        if (fused) {
             const volatile double *hp = hpart;
             for (int k = 0; k < nfused; k++) rn += hp[k];  // fixed order: deterministic
         } else {
@@ -541,7 +552,7 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             *conv = true;
             break;
         }
-        if (!ahead && it < max_iter) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, nfused > 0);
+        if (!ahead && it < max_iter) n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, false, st, fused, wz);
         prev2 = prev;
         prev = rn;
         if (stall_stop) {
@@ -582,9 +593,11 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         P.pA = dalloc<double>(n);
         P.pB = dalloc<double>(n);
         P.q = dalloc<double>(n);
+        P.qB = dalloc<double>(n);
+        P.w = dalloc<double>(n);
         P.wD = dalloc<double>(M + 2);
         P.sc = dalloc<double>(NPSCAL);
-        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc)
+        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.qB || !P.w || !P.wD || !P.sc)
             return VVM_ERR_NOMEM;
         hipMemcpy(P.wD, wd.data(), sizeof(double) * (M + 2), hipMemcpyHostToDevice);
     }
@@ -602,9 +615,11 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         P.pA = dalloc<float>(n);
         P.pB = dalloc<float>(n);
         P.q = dalloc<float>(n);
+        P.qB = dalloc<float>(n);
+        P.w = dalloc<float>(n);
         P.wD = dalloc<float>(M + 2);
         P.sc = dalloc<double>(NPSCAL);
-        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.wD || !P.sc)
+        if (!P.x || !P.r || !P.rB || !P.zA || !P.zB || !P.pA || !P.pB || !P.q || !P.qB || !P.w || !P.wD || !P.sc)
             return VVM_ERR_NOMEM;
         std::vector<float> wf(wd.begin(), wd.end());
         hipMemcpy(P.wD, wf.data(), sizeof(float) * (M + 2), hipMemcpyHostToDevice);
@@ -634,7 +649,7 @@ template <typename T>
 static void free_pcg(PCG<T> &P) {
     free_hier(P.mg);
     for (void *p : {(void *)P.x, (void *)P.r, (void *)P.rB, (void *)P.zA, (void *)P.zB, (void *)P.pA,
-                    (void *)P.pB, (void *)P.q, (void *)P.wD, (void *)P.sc})
+                    (void *)P.pB, (void *)P.q, (void *)P.qB, (void *)P.w, (void *)P.wD, (void *)P.sc})
         if (p) hipFree(p);
     P = PCG<T>();
 }

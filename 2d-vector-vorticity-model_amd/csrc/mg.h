@@ -58,27 +58,44 @@ int mg_plan(MGHier<T> &H, size_t lds_budget);
 // the result moves between buffers).  With init it also stores r.z in
 // sc[P_RZ]; otherwise it leaves the r.z partials in row 0 of rs for the next
 // k_pcg_pq and returns their count.  With skip_down0 the level-0 down step
-// has already run (mg_down0_cg).
+// has already run (mg_down0_cg).  With w (the single-reduction CG), the
+// level-0 up step also writes w = L z and leaves the partials of r.z and w.z
+// in rows 0 and 1 of rs, and nothing is finalized, init or not.
 template <typename T>
 int mg_vcycle(MGHier<T> &H, const T *r, T *&zA, T *&zB, const RedSlot &rs, double *sc, bool init, hipStream_t st,
-              bool skip_down0 = false);
+              bool skip_down0 = false, T *w = nullptr);
 
-// The CG update fused into the level-0 down step: alpha = sc[rz] / p.q
-// (p.q from the npq partials in rs_pq), rout = rin - alpha q and x += alpha p
-// on the points each workgroup owns, the smoothing and restriction on the updated
-// r, and ||w r||^2 per workgroup into hpart (host-mapped memory; the host
-// adds the nblocks entries).  Returns nblocks, or 0 when the hierarchy has
-// no tiled level 0 (the caller then runs the update separately).
+// Whether the hierarchy has the tiled level 0 that mg_down0_cg and the
+// w-writing up step need, within max_blocks workgroups each.
+template <typename T>
+bool mg_cg_fusable(const MGHier<T> &H, int max_blocks);
+
+// One iteration of the single-reduction (Chronopoulos-Gear) CG fused into
+// the level-0 down step.  From gamma = r.z and delta = w.z (the partials the
+// last V-cycle's level-0 up step left; z = M r, w = L z):
+//   beta = gamma / gamma_old (0 on the first iteration),
+//   alpha = gamma / (delta - beta gamma / alpha_old) (gamma / delta first),
+//   p = z + beta p, q = w + beta q (= L p), x += alpha p, r -= alpha q,
+// then the smoothing and restriction of the new r, and ||wD r||^2 per
+// workgroup into hpart (host-mapped memory; the host adds the nblocks
+// entries).  In exact arithmetic these are the iterates of standard PCG,
+// with one grid-wide reduction an iteration instead of two.  Returns
+// nblocks, or 0 when the hierarchy is not fusable.
 template <typename T>
 struct CGFuse {
     T *x;
     const T *rin;  // r before the update (read with halos by every workgroup)
     T *rout;       // r after it (another buffer: the reads above must see the old r)
-    const T *p, *q, *wD;
-    const double *sc;
-    int rz;  // slot of r.z in sc
-    RedSlot rs_pq;
-    int npq;
+    T *p;          // search direction, updated in place on owned points only
+    const T *qin;  // L p before the update (read with halos)
+    T *qout;       // after it (another buffer, as for r)
+    const T *w;    // L z of the last V-cycle
+    const T *wD;
+    double *sc;
+    int g_old, g_new, a_old, a_new;  // slots of gamma and alpha in sc, last and this iteration
+    int first;
+    RedSlot rs_gd;  // gamma partials in row 0, delta partials in row 1
+    int ngd;
     double *hpart;
 };
 template <typename T>

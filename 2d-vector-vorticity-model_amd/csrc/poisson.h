@@ -25,6 +25,8 @@ struct PCG {
     MGHier<T> mg;
     T *x = nullptr, *r = nullptr, *zA = nullptr, *zB = nullptr, *pA = nullptr, *pB = nullptr, *q = nullptr;
     T *rB = nullptr;       // the other residual buffer of the fused update (swapped with r)
+    T *qB = nullptr;       // the other L p buffer of the single-reduction CG (swapped with q)
+    T *w = nullptr;        // L z of the last V-cycle (single-reduction CG)
     T *wD = nullptr;       // row weights rhow[k] for ||D r||, [M+2]
     double *sc = nullptr;  // NPSCAL device scalars
 };
