@@ -191,6 +191,54 @@ __device__ inline double thermo_tend(const Grid &G, const double *f, const doubl
     return -px - pz;
 }
 
+// The parts of a scalar face flux that depend only on the face (density and
+// velocity), formed once and applied to every field advected across it.
+// flux(f) rounds exactly as sflux_x / sflux_z do: the same products in the
+// same order (upwind3's rho0 p0 (f0 - fm1) is (rho0 p0) (f0 - fm1), and so on).
+struct FaceFlux {
+    double rv;          // r v of the centred term
+    double a, b, c, d;  // rho0 p0, sqrt(rho0 p0 rhom pm), rho0 m0, sqrt(|rho0 m0 rhop mp|)
+    bool corr;          // inside the third-order correction window
+    size_t s;           // index step across the face (1 for x faces, the row pitch for z faces)
+    __device__ double flux(const double *__restrict__ f, size_t p) const {
+        const double f0 = f[p], fm1 = f[p - s];
+        double fl = rv * (f0 + fm1);
+        if (corr) {
+            const double fm2 = f[p - 2 * s], fp1 = f[p + s];
+            fl += -1. / 3. * (a * (f0 - fm1) - b * (fm1 - fm2) - c * (f0 - fm1) - d * (fp1 - f0));
+        }
+        return fl;
+    }
+};
+__device__ inline FaceFlux face_coeffs(double rho0, double v0, double rhom, double vm, double rhop, double vp,
+                                       bool corr, size_t s) {
+    FaceFlux F;
+    F.rv = rho0 * v0;
+    F.corr = corr;
+    F.s = s;
+    F.a = F.b = F.c = F.d = 0.;
+    if (corr) {
+        const double p0 = plus_(v0), pm = plus_(vm), m0 = minu_(v0), mp = minu_(vp);
+        F.a = rho0 * p0;
+        F.b = sqrt(F.a * rhom * pm);
+        F.c = rho0 * m0;
+        F.d = sqrt(fabs(F.c * rhop * mp));
+    }
+    return F;
+}
+// x face between i-1 and i / z face between k-1 and k, as sflux_x / sflux_z
+__device__ inline FaceFlux face_x(const Grid &G, const double *u, const double *rhou, int i, int k) {
+    const bool corr = corr_window(G, i, k);
+    const double r = rhou[k];
+    return face_coeffs(r, u[ix(G, i, k)], r, corr ? u[ix(G, i - 1, k)] : 0., r, corr ? u[ix(G, i + 1, k)] : 0., corr,
+                       1);
+}
+__device__ inline FaceFlux face_z(const Grid &G, const double *w, const double *rhow, int i, int k) {
+    const bool corr = corr_window(G, i, k);
+    return face_coeffs(rhow[k], w[ix(G, i, k)], corr ? rhow[k - 1] : 0., corr ? w[ix(G, i, k - 1)] : 0.,
+                       corr ? rhow[k + 1] : 0., corr ? w[ix(G, i, k + 1)] : 0., corr, G.pitch);
+}
+
 // Rain sedimentation flux of Advection_qrVT (Advection.cpp:165-166) at (i,k).
 __device__ inline double vt_flux(const Grid &G, const double *qr, const double *rhow, const double *vtfac, int i,
                                  int k) {
@@ -214,7 +262,9 @@ __device__ inline double vt_flux(const Grid &G, const double *qr, const double *
 // and powers, twice.)
 constexpr int T4X = 64, T4Y = 8;
 
-__global__ void __launch_bounds__(T4X *T4Y) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
+// Four workgroups a CU (the LDS allows four), so that 1024 of them cover
+// 2048 x 256 in one round: eight waves a SIMD, at most 64 VGPRs.
+__global__ void __launch_bounds__(T4X *T4Y) __attribute__((amdgpu_waves_per_eu(8))) k_thermo4(Grid G, StepPtrs S, int step, RedSlot rs) {
     __shared__ double Fx[4][T4Y][T4X + 1];  // x-face flux at faces i0 .. i0+T4X (west faces)
     __shared__ double Fz[5][T4Y + 1][T4X];  // z-face flux (4 fields, rain fall) at faces k0 .. k0+T4Y
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -226,27 +276,45 @@ __global__ void __launch_bounds__(T4X *T4Y) k_thermo4(Grid G, StepPtrs S, int st
     const bool row_in = (k >= 1 && k <= nz - 2), inside = row_in && i >= 1 && i <= nx - 2;
     // the ghost column nx-1 holds the face east of column nx-2: the
     // periodic image of face 1
+    // the density-velocity part of each face is formed once for all four
+    // fields (sflux_x / sflux_z per field would repeat its square roots)
     if (row_in && i >= 1 && i <= nx - 1) {
+        const int ii = (i == nx - 1) ? 1 : i;
+        const FaceFlux F = face_x(G, u, rhou, ii, k);
+        const size_t p = ix(G, ii, k);
 #pragma unroll
-        for (int q = 0; q < 4; q++) Fx[q][ty][tx] = sflux_x(G, fl[q], u, rhou, (i == nx - 1) ? 1 : i, k);
+        for (int q = 0; q < 4; q++) Fx[q][ty][tx] = F.flux(fl[q], p);
     }
     if (inside) {
+        const FaceFlux F = face_z(G, w, rhow, i, k);
+        const size_t p = ix(G, i, k);
 #pragma unroll
-        for (int q = 0; q < 4; q++) Fz[q][ty][tx] = sflux_z(G, fl[q], w, rhow, i, k);
+        for (int q = 0; q < 4; q++) Fz[q][ty][tx] = F.flux(fl[q], p);
         Fz[4][ty][tx] = vt_flux(G, fl[3], rhow, S.P.vtfac, i, k);
     }
-    // edge faces, one kind per wave: waves 0-3 the top faces of field ty,
-    // wave 4 the top rain-fall faces, wave 5 the east faces (row, field)
+    // edge faces: waves 0-1 the top faces (four fields each), wave 2 the
+    // top rain-fall faces, wave 3 the east faces (one row each)
     {
         const int kt = k0 + T4Y, ie = i0 + T4X;
-        if (ty < 5) {
-            if (i >= 1 && i <= nx - 2 && kt <= nz - 2)
-                Fz[ty][T4Y][tx] = (ty < 4) ? sflux_z(G, fl[ty], w, rhow, i, kt) : vt_flux(G, fl[3], rhow, S.P.vtfac, i, kt);
-        } else if (ty == 5 && tx < 4 * T4Y) {
-            const int r = tx % T4Y, q = tx / T4Y, kr = k0 + r;
+        if (ty == 0) {
+            if (i >= 1 && i <= nx - 2 && kt <= nz - 2) {
+                const FaceFlux F = face_z(G, w, rhow, i, kt);
+                const size_t p = ix(G, i, kt);
+#pragma unroll
+                for (int q = 0; q < 4; q++) Fz[q][T4Y][tx] = F.flux(fl[q], p);
+            }
+        } else if (ty == 2) {
+            if (i >= 1 && i <= nx - 2 && kt <= nz - 2) Fz[4][T4Y][tx] = vt_flux(G, fl[3], rhow, S.P.vtfac, i, kt);
+        } else if (ty == 3 && tx < T4Y) {
+            const int kr = k0 + tx;
             // the face east of column nx-2 is the periodic image of face 1
-            if (ie - 1 <= nx - 2 && kr >= 1 && kr <= nz - 2)
-                Fx[q][r][T4X] = sflux_x(G, fl[q], u, rhou, (ie == nx - 1) ? 1 : ie, kr);
+            if (ie - 1 <= nx - 2 && kr >= 1 && kr <= nz - 2) {
+                const int ii = (ie == nx - 1) ? 1 : ie;
+                const FaceFlux F = face_x(G, u, rhou, ii, kr);
+                const size_t p = ix(G, ii, kr);
+#pragma unroll
+                for (int q = 0; q < 4; q++) Fx[q][tx][T4X] = F.flux(fl[q], p);
+            }
         }
     }
     __syncthreads();
