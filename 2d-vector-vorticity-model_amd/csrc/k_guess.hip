@@ -83,9 +83,12 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
     }
     if (!dst) dst = gfeat;  // in place: each element reads and writes its own slot
     const dim3 g = ggrid(N, M);
+    // rows 2-4 of the reduction slots: row 1 may hold the pending ||b||^2
+    // partials of the right-hand side
+    const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
     hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
-                       pcg64.mg.lev[0].cu, gfeat, red);
-    finalize<3>(red, nblocks(g), gsc, RED_STORE, 0, st);
+                       pcg64.mg.lev[0].cu, gfeat, rf);
+    finalize<3>(rf, nblocks(g), gsc, RED_STORE, 0, st);
     hipLaunchKernelGGL(k_guess_normalize, dim3(unsigned((3 * plane + 255) / 256)), dim3(256), 0, st, plane,
                        double(plane), gsc, gfeat, dst);
 }
@@ -101,9 +104,12 @@ bool Poisson::learned_guess(hipStream_t st) {
     // scales them, its last writes the guess into x (rows 1..M; the
     // Dirichlet rows of x stay zero)
     const dim3 g = ggrid(N, M);
+    // rows 2-4 of the reduction slots: row 1 may hold the pending ||b||^2
+    // partials of the right-hand side
+    const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
     hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
-                       pcg64.mg.lev[0].cu, gfeat, red);
-    finalize<3>(red, nblocks(g), gsc, RED_STORE, 0, st);
+                       pcg64.mg.lev[0].cu, gfeat, rf);
+    finalize<3>(rf, nblocks(g), gsc, RED_STORE, 0, st);
     const double npts = double(plane);
     const NetGeo geo{1, M, N, 1};
     const NetFeatIn fin{gfeat, gsc, npts};

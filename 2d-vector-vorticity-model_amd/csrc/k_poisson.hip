@@ -98,45 +98,53 @@ __global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r,
     block_partials<1>(part, rs);
 }
 
-// r = f - L x and ||w r||^2 into sc[slot] (and sc[slot2] if >= 0).
-template <typename T, typename TF>
-__global__ void __launch_bounds__(LX *LY) k_residual(int N, int M, const TF *f, const T *x, T *r, const T *cx,
-                                                     const T *cu, const T *wD, double *sc, RedSlot rs, int slot,
-                                                     int slot2) {
+// The solve's first residual with the guess or the refinement correction
+// folded in.  XF_EXTRAP: x = 2 x1 - x2 (the extrapolated guess), XF_ZERO:
+// x = 0, XF_CORR: x = xin + e32 (the FP32 correction; written to another
+// buffer, since neighbouring workgroups read xin), XF_NONE: x as it is.
+// Every value of x the stencil needs is formed on the fly with the same
+// arithmetic the separate kernels used, so r and x are bit for bit what they
+// gave.  With r32 the FP32 right-hand side of the next correction (r as
+// float, x32 = 0) is written too.
+enum { XF_NONE = 0, XF_EXTRAP, XF_ZERO, XF_CORR };
+template <int XF>
+__global__ void __launch_bounds__(LX *LY) k_residual_x(int N, int M, const double *f, const double *xin,
+                                                       double *xout, const double *x1, const double *x2,
+                                                       const float *e32, double *r, const double *cx,
+                                                       const double *cu, const double *wD, RedSlot rs, float *r32,
+                                                       float *x32) {
     const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
+    auto xv = [&](size_t a) -> double {
+        if (XF == XF_EXTRAP) return 2. * x1[a] - x2[a];
+        if (XF == XF_ZERO) return 0.;
+        if (XF == XF_CORR) return xin[a] + double(e32[a]);
+        return xin[a];
+    };
     double part[1] = {0.};
     if (i < N && j <= M) {
         const size_t c = size_t(j) * N + i;
-        const T rv = T(f[c]) - applyL(x, N, i, j, cx[j], cu[j], cu[j - 1]);
+        const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
+        const size_t rw = size_t(j) * N;
+        const double xc = xv(rw + i);
+        const double Lx = cx[j] * (2. * xc - xv(rw + im) - xv(rw + ip)) + cu[j] * (xc - xv(rw + N + i)) +
+                          cu[j - 1] * (xc - xv(rw - N + i));
+        const double rv = f[c] - Lx;
         r[c] = rv;
-        const double wr = double(wD[j]) * double(rv);
+        if (XF != XF_NONE) {
+            xout[c] = xc;
+            // the boundary rows: XF_EXTRAP and XF_ZERO form them as the
+            // separate kernels did, XF_CORR carries them over
+            if (j == 1) xout[i] = (XF == XF_CORR) ? xin[i] : xv(i);
+            if (j == M) xout[rw + N + i] = (XF == XF_CORR) ? xin[rw + N + i] : xv(rw + N + i);
+        }
+        if (r32) {
+            r32[c] = float(rv);
+            x32[c] = 0.f;
+        }
+        const double wr = wD[j] * rv;
         part[0] = wr * wr;
     }
     block_partials<1>(part, rs);
-}
-
-// FP64 residual -> FP32 right-hand side of the correction equation, with
-// its weighted norm in the FP32 solver's scalars.
-__global__ void __launch_bounds__(LX *LY) k_to_fp32(int N, int M, const double *r64, float *r32, float *x32,
-                                                    const float *wD, double *sc32, RedSlot rs) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    double part[1] = {0.};
-    if (i < N && j <= M) {
-        const size_t c = size_t(j) * N + i;
-        const float v = float(r64[c]);
-        r32[c] = v;
-        x32[c] = 0.f;
-        const double wr = double(wD[j]) * double(v);
-        part[0] = wr * wr;
-    }
-    block_partials<1>(part, rs);
-}
-
-__global__ void __launch_bounds__(LX *LY) k_add_correction(int N, int M, double *x64, const float *e32) {
-    const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
-    if (i >= N || j > M) return;
-    const size_t c = size_t(j) * N + i;
-    x64[c] += double(e32[c]);
 }
 
 // ------------------------------------------------------------- cal_w glue
@@ -624,12 +632,14 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         std::vector<float> wf(wd.begin(), wd.end());
         hipMemcpy(P.wD, wf.data(), sizeof(float) * (M + 2), hipMemcpyHostToDevice);
         r64 = dalloc<double>(n);
-        if (!r64) return VVM_ERR_NOMEM;
+        x32B = dalloc<float>(n);
+        if (!r64 || !x32B) return VVM_ERR_NOMEM;
     }
     f = dalloc<double>(n);
     xprev = dalloc<double>(n);
     xprev2 = dalloc<double>(n);
-    if (!f || !xprev || !xprev2) return VVM_ERR_NOMEM;
+    x64B = dalloc<double>(n);
+    if (!f || !xprev || !xprev2 || !x64B) return VVM_ERR_NOMEM;
     // pinned, coherent and mapped: k_finalize writes the CG residual norm
     // here directly, and the host reads it after an event
     if (hipHostMalloc(reinterpret_cast<void **>(&hscal), 16 * sizeof(double),
@@ -657,11 +667,14 @@ static void free_pcg(PCG<T> &P) {
 void Poisson::destroy() {
     free_pcg(pcg64);
     if (have32) free_pcg(pcg32);
-    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)r64, (void *)gfeat,
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)x64B, (void *)x32B, (void *)r64, (void *)gfeat,
                     (void *)gsc, (void *)samples})
         if (p) hipFree(p);
-    f = xprev = xprev2 = r64 = nullptr;
-    gfeat = samples = nullptr;
+    f = xprev = xprev2 = r64 = x64B = nullptr;
+    gfeat = samples = x32B = nullptr;
+    bn_pending = 0;
+    x_pending = 0;
+    corr_pending = false;
     gsc = nullptr;
     gscratch.release();
     sample_cap = sample_count = 0;
@@ -683,15 +696,68 @@ int Poisson::replan(size_t budget) {
     return VVM_OK;
 }
 
+// ||b||^2 partials go to row 1 of red; the first residual's reduction adds
+// them up with its own (flush_pending, if something needs red first)
 void Poisson::rhs_from_zetap(const Grid &G, const double *zetap, const double *rhow_dev, hipStream_t st) {
-    hipLaunchKernelGGL(k_calw_rhs, lgrid(N, M), dim3(LX, LY), 0, st, G, zetap, rhow_dev, f, pcg64.sc, red);
-    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st, -1, hscal_dev + P_BNORM2);
+    const RedSlot row1{red.partial + red.max_blocks, red.max_blocks};
+    hipLaunchKernelGGL(k_calw_rhs, lgrid(N, M), dim3(LX, LY), 0, st, G, zetap, rhow_dev, f, pcg64.sc, row1);
+    bn_pending = nblocks(lgrid(N, M));
 }
 
 void Poisson::rhs_from_b(const double *b, hipStream_t st) {
     // the FP64 row weights wD[j] are rhow[j+1]
-    hipLaunchKernelGGL(k_rhs_from_b, lgrid(N, M), dim3(LX, LY), 0, st, N, M, b, pcg64.wD, f, pcg64.sc, red);
-    finalize<1>(red, nblocks(lgrid(N, M)), pcg64.sc, RED_STORE, P_BNORM2, st, -1, hscal_dev + P_BNORM2);
+    const RedSlot row1{red.partial + red.max_blocks, red.max_blocks};
+    hipLaunchKernelGGL(k_rhs_from_b, lgrid(N, M), dim3(LX, LY), 0, st, N, M, b, pcg64.wD, f, pcg64.sc, row1);
+    bn_pending = nblocks(lgrid(N, M));
+}
+
+void Poisson::flush_pending(hipStream_t st) {
+    if (bn_pending) {
+        const RedSlot row1{red.partial + red.max_blocks, red.max_blocks};
+        finalize<1>(row1, bn_pending, pcg64.sc, RED_STORE, P_BNORM2, st, -1, hscal_dev + P_BNORM2);
+        bn_pending = 0;
+    }
+    const size_t n = size_t(M + 2) * N;
+    if (x_pending == XF_ZERO)
+        hipMemsetAsync(pcg64.x, 0, n * sizeof(double), st);
+    else if (x_pending == XF_EXTRAP)
+        hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, n, pcg64.x, xprev, xprev2);
+    x_pending = XF_NONE;
+}
+
+void Poisson::first_residual(double *r, bool fp32_rhs, hipStream_t st) {
+    const dim3 g = lgrid(N, M), b(LX, LY);
+    PCG<double> &D = pcg64;
+    const float *e32 = have32 ? pcg32.x : nullptr;
+    const int xf = corr_pending ? int(XF_CORR) : x_pending;
+    double *xo = (xf == XF_CORR) ? x64B : D.x;
+    // the FP32 copy of r, and the FP32 x zeroed for the next correction; the
+    // correction being applied is read from pcg32.x, so the zeros then go to
+    // the other FP32 buffer
+    float *r32 = fp32_rhs ? pcg32.r : nullptr;
+    float *x32 = fp32_rhs ? ((xf == XF_CORR) ? x32B : pcg32.x) : nullptr;
+#define VVM_RESIDUAL_X(XF)                                                                                        \
+    hipLaunchKernelGGL((k_residual_x<XF>), g, b, 0, st, N, M, f, D.x, xo, xprev, xprev2, e32, r, D.mg.lev[0].cx, \
+                       D.mg.lev[0].cu, D.wD, red, r32, x32)
+    switch (xf) {
+        case XF_EXTRAP: VVM_RESIDUAL_X(XF_EXTRAP); break;
+        case XF_ZERO: VVM_RESIDUAL_X(XF_ZERO); break;
+        case XF_CORR: VVM_RESIDUAL_X(XF_CORR); break;
+        default: VVM_RESIDUAL_X(XF_NONE); break;
+    }
+#undef VVM_RESIDUAL_X
+    if (xf == XF_CORR) std::swap(D.x, x64B);
+    if (fp32_rhs && xf == XF_CORR) std::swap(pcg32.x, x32B);
+    x_pending = XF_NONE;
+    corr_pending = false;
+    // the norms land in mapped host memory (hscal) directly: no copy in the stream
+    if (bn_pending) {
+        // rows 0 (r) and 1 (b), same grid: sc[P_RNORM2], sc[P_BNORM2]
+        finalize<2>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st, -1, hscal_dev + P_RNORM2);
+        bn_pending = 0;
+    } else {
+        finalize<1>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st, -1, hscal_dev + P_RNORM2);
+    }
 }
 
 void Poisson::x_to_ref(double *xref, hipStream_t st) {
@@ -708,17 +774,19 @@ void Poisson::back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStr
 }
 
 void Poisson::set_guess(int mode, hipStream_t st) {
-    const size_t n = size_t(M + 2) * N;
-    if (mode == VVM_GUESS_ZERO) {
-        hipMemsetAsync(pcg64.x, 0, n * sizeof(double), st);
-    } else if (mode == VVM_GUESS_EXTRAP && nsolves >= 2) {
-        hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, n, pcg64.x, xprev, xprev2);
-    }
+    (void)st;
+    // formed by the solve's first residual kernel (first_residual)
+    x_pending = XF_NONE;
+    if (mode == VVM_GUESS_ZERO)
+        x_pending = XF_ZERO;
+    else if (mode == VVM_GUESS_EXTRAP && nsolves >= 2)
+        x_pending = XF_EXTRAP;
     // VVM_GUESS_PREV (and the learned guess, written by its own kernels)
     // start from the current contents of x.
 }
 
 void Poisson::remember_solution(hipStream_t st) {
+    flush_pending(st);  // nothing is pending after a solve; a guess set without one is formed now
     const size_t n = size_t(M + 2) * N;
     std::swap(xprev, xprev2);
     hipMemcpyAsync(xprev, pcg64.x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
@@ -728,11 +796,7 @@ void Poisson::remember_solution(hipStream_t st) {
 SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     SolveResult R;
     PCG<double> &P = pcg64;
-    const dim3 g = lgrid(N, M), b(LX, LY);
-    hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, P.x, P.r, P.mg.lev[0].cx, P.mg.lev[0].cu,
-                       P.wD, P.sc, red, int(P_RNORM2), int(P_RNORM2_0));
-    // the norms land in mapped host memory (hscal) directly: no copy in the stream
-    finalize<1>(red, nblocks(g), P.sc, RED_STORE, P_RNORM2, st, P_RNORM2_0, hscal_dev + P_RNORM2);
+    first_residual(P.r, false, st);
     hipStreamSynchronize(st);
     const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
     if (bn2 == 0.) {
@@ -741,6 +805,7 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
         return R;
     }
     R.rel0 = std::sqrt(rn2 / bn2);
+    (void)P;
     const bool parity = (tol <= 0.);
     const double target = parity ? 1e-30 * bn2 : tol * tol * bn2;
     if (!(rn2 > target)) {
@@ -764,12 +829,10 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
     if (!have32) return solve_fp64(tol, max_inner * max_outer, st);
     PCG<double> &D = pcg64;
     PCG<float> &F = pcg32;
-    const dim3 g = lgrid(N, M), b(LX, LY);
     double bn2 = -1.;
     for (int outer = 0; outer <= max_outer; outer++) {
-        hipLaunchKernelGGL((k_residual<double, double>), g, b, 0, st, N, M, f, D.x, r64, D.mg.lev[0].cx,
-                           D.mg.lev[0].cu, D.wD, D.sc, red, int(P_RNORM2), -1);
-        finalize<1>(red, nblocks(g), D.sc, RED_STORE, P_RNORM2, st, -1, hscal_dev + P_RNORM2);
+        // r64, and its FP32 copy for the correction solve (unused if converged)
+        first_residual(r64, true, st);
         hipStreamSynchronize(st);
         bn2 = hscal[P_BNORM2];
         const double rn2 = hscal[P_RNORM2];
@@ -787,7 +850,6 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         }
         if (outer == max_outer) break;
         R.refine++;
-        hipLaunchKernelGGL(k_to_fp32, g, b, 0, st, N, M, r64, F.r, F.x, F.wD, F.sc, red);
         // The inner solve reduces its residual just enough to meet tol (with
         // a margin for the FP64 residual after the correction), by no more
         // than FP32 resolves and by at least inner_tol: one refinement when
@@ -801,8 +863,9 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         bool conv = false;
         R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, r0, &last,
                             &conv);
-        hipLaunchKernelGGL(k_add_correction, g, b, 0, st, N, M, D.x, F.x);
+        corr_pending = true;  // x += e, in the next sweep's residual kernel
     }
+    (void)D;
     return R;
 }
 

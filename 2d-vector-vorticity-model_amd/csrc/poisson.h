@@ -110,6 +110,22 @@ class Poisson {
     NetScratch gscratch;       // this solver's network activations
     void guess_features(float *dst, hipStream_t st);
     hipEvent_t ev = nullptr;
+    // Work folded into the next solve's first residual kernel (one launch
+    // instead of two each): the ||b||^2 partials the right-hand-side kernel
+    // left in row 1 of red (their block count, 0 for none), the guess
+    // (GuessMode below) and, between refinement sweeps, the FP32
+    // correction.  flush_pending runs the first two on their own, for
+    // callers that use red or x before the solve.
+    int bn_pending = 0;
+    int x_pending = 0;
+    bool corr_pending = false;
+    double *x64B = nullptr;  // the other FP64 solution buffer (the correction writes it)
+    float *x32B = nullptr;   // the other FP32 solution buffer (zeroed while the correction reads x)
+    void flush_pending(hipStream_t st);
+    // r = f - L x (x with the pending guess or correction applied, and
+    // written), ||w r||^2 (and ||b||^2 if pending) into sc and hscal; with
+    // fp32_rhs also the FP32 copy of r and a zero FP32 x.
+    void first_residual(double *r, bool fp32_rhs, hipStream_t st);
 };
 
 // cal_u on the device (src/PoissonSolver.cpp:580-653): top-row potential
