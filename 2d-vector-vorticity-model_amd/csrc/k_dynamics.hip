@@ -383,7 +383,11 @@ __global__ void __launch_bounds__(BX *BY) k_scale_forcing(Grid G, StepPtrs S, in
     }
     double th = S.fut[F_TH][p];
     if (forcing) {
+        // AddForcing's order (AddForcing.cpp:15-25): large-scale forcing,
+        // radiation heating [K/day], RADIATIONCOOLING, the random kick
         if (tropical) th += G.dt * S.P.Q1LS[k];
+        if (S.rad) th += G.dt * S.rad[p] / 86400.;
+        if (S.radcool) th += G.dt * (-2. / 86400.);
         if (random_kick) th += G.dt * S.forcing[p];
     }
     store_center(S.fut[F_TH], G, i, k, th);

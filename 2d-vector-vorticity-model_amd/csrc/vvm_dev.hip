@@ -63,6 +63,10 @@ struct vvm_dev {
     // qv, qc, qr, the microphysics rates and the negval reduction slots and
     // scalars, none of which B touches
     hipStream_t side = nullptr;
+    // host-supplied radiation heating (vvm_dev_set_radiation)
+    double *rad = nullptr;
+    bool rad_on = false;
+    int radcool = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mp[2] = {};
 
     // per-kernel event timer (vvm_dev_kernel_timer)
@@ -142,6 +146,8 @@ StepPtrs ptrs(vvm_dev *h) {
     S.RKM = h->RKM;
     S.RKH = h->RKH;
     S.forcing = h->forcing;
+    S.rad = h->rad_on ? h->rad : nullptr;
+    S.radcool = h->radcool;
     S.evap = h->evap;
     S.accr = h->accr;
     S.autoc = h->autoc;
@@ -844,6 +850,23 @@ int vvm_dev_collect(vvm_dev_t *h, int capacity) {
 }
 
 int vvm_dev_collected(vvm_dev_t *h) { return (h && h->pois_ready) ? h->pois.sample_count : 0; }
+
+int vvm_dev_set_radiation(vvm_dev_t *h, const double *heating_rate, int cooling) {
+    if (!h || (cooling != 0 && cooling != 1)) return VVM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    HIPCHK(h, hipStreamSynchronize(h->st));  // no step still reads the old rate
+    if (heating_rate) {
+        if (!h->rad) {
+            h->rad = dev_alloc(h, size_t(h->G.nz) * h->G.pitch);
+            if (!h->rad) return fail(h, VVM_ERR_NOMEM, "radiation rate");
+        }
+        const int rc = put2d(h, h->rad, heating_rate);
+        if (rc) return rc;
+    }
+    h->rad_on = heating_rate != nullptr;
+    h->radcool = cooling;
+    return VVM_OK;
+}
 
 int vvm_dev_set_shape(vvm_dev_t *h, int cw, int kib) {
     if (!h || cw < 0 || kib < 0 || kib > 64) return VVM_ERR_ARG;

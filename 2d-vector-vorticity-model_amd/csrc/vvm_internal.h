@@ -45,6 +45,8 @@ struct StepPtrs {
     double *now[NPROG], *past[NPROG], *fut[NPROG];
     double *hist[NHIST][2];
     double *u, *w, *RKM, *RKH, *forcing;
+    const double *rad;  // radiation heating rate [K/day] (nullptr: none)
+    int radcool;        // RADIATIONCOOLING: -2 K/day on theta
     double *evap, *accr, *autoc, *cond;
     double *precip, *xi, *uxi;
     double *s;  // device scalars

@@ -146,7 +146,8 @@ void vvm::allocateMemory() {
         {&qc, &qccont}, {&qcm, &qcmcont}, {&qrp, &qrpcont}, {&qr, &qrcont}, {&qrm, &qrmcont},
         {&u, &ucont}, {&w, &wcont}, {&RKM, &RKMcont}, {&RKH, &RKHcont}, {&evaporation, &evaporationcont},
         {&accretion, &accretioncont}, {&autoconversion, &autoconversioncont},
-        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont}};
+        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont},
+        {&radiation_heating_rate, &radiation_heating_ratecont}};
     for (auto &f : f2) *f.first = allocate2DContinuousArray(nx, nz, *f.second);
     std::pair<double ****, double **> f3[] = {
         {&dth_advect, &dth_advectcont}, {&dth_buoyancy, &dth_buoyancycont}, {&dzeta_advect, &dzeta_advectcont},
@@ -171,7 +172,8 @@ void vvm::deallocateMemory() {
         {&qc, &qccont}, {&qcm, &qcmcont}, {&qrp, &qrpcont}, {&qr, &qrcont}, {&qrm, &qrmcont},
         {&u, &ucont}, {&w, &wcont}, {&RKM, &RKMcont}, {&RKH, &RKHcont}, {&evaporation, &evaporationcont},
         {&accretion, &accretioncont}, {&autoconversion, &autoconversioncont},
-        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont}};
+        {&condensation, &condensationcont}, {&init_th_forcing, &init_th_forcingcont},
+        {&radiation_heating_rate, &radiation_heating_ratecont}};
     for (auto &f : f2) {
         deallocate2DContinuousArray(*f.first, *f.second);
         *f.first = nullptr;
@@ -769,13 +771,21 @@ void vvm::Iteration::TimeMarching(vvm &model) {
     model.dev_stale = true;
     const int nmax = int(model.TIMEEND / model.dt);
     ensure_device(model);
+    const bool radiation = static_cast<bool>(model.solve_radiation) && model.radiation_every > 0;
+    auto push_radiation = [&]() {
+        const int rc = vvm_dev_set_radiation(model.dev, model.radiation_heating_ratecont, model.radiation_cooling ? 1 : 0);
+        check(model, rc, "vvm_dev_set_radiation");
+    };
+    if (radiation || model.radiation_cooling) push_radiation();  // the rate as it stands (zero at a cold start)
     while (model.step < nmax) {
         if (model.OUTPUTSTEP > 0 && is_output_step(model, model.step) && !model.outputpath.empty()) {
             if (model.output_nc_files) vvm::Output::output_nc(model.step, model);
             if (model.output_state_files) vvm::Output::output_state(model.step, model);
         }
         int next = model.step + 1;
-        while (next < nmax && !(model.OUTPUTSTEP > 0 && is_output_step(model, next))) next++;
+        while (next < nmax && !(model.OUTPUTSTEP > 0 && is_output_step(model, next)) &&
+               !(radiation && next % model.radiation_every == 0))
+            next++;
         const int n = next - model.step;
         vvm_step_stats st{};
         int rc = vvm_dev_step(model.dev, n, &st);
@@ -796,6 +806,24 @@ void vvm::Iteration::TimeMarching(vvm &model) {
             model.t_diffusion[slot] = st.t_diffusion;
         }
         model.step = next;
+        if (radiation && next % model.radiation_every == 0) {
+            // the reference calls solve_radiation before nextTimeStep, on the
+            // step's current level, which the device step has since rotated
+            // into the m arrays
+            model.sync_to_host();
+            std::swap(model.th, model.thm);
+            std::swap(model.qv, model.qvm);
+            std::swap(model.qc, model.qcm);
+            std::swap(model.qr, model.qrm);
+            std::swap(model.zeta, model.zetam);
+            model.solve_radiation(model);
+            std::swap(model.th, model.thm);
+            std::swap(model.qv, model.qvm);
+            std::swap(model.qc, model.qcm);
+            std::swap(model.qr, model.qrm);
+            std::swap(model.zeta, model.zetam);
+            push_radiation();
+        }
     }
     if (model.print_steps) std::cout.flush();
     model.sync_to_host();

@@ -165,6 +165,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_ens_tune": (C.c_int, [C.c_void_p, C.c_int]),
         "vvm_ens_step": (C.c_int, [C.c_void_p, C.c_int, P(VvmEnsStats)]),
         "vvm_dev_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+        "vvm_dev_set_radiation": (C.c_int, [C.c_void_p, PTR, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name, None)
@@ -443,6 +444,17 @@ class Device:
 
     def set_shape(self, cw: int = 0, coarse_kib: int = 0) -> None:
         self._check(self.lib.vvm_dev_set_shape(self.h, cw, coarse_kib), "set_shape")
+
+    def set_radiation(self, heating_rate: np.ndarray | None, cooling: bool = False) -> None:
+        """The radiation heating rate [K/day], (nx, nz) in the reference's
+        layout, added to thp by every later step (None removes it); cooling
+        adds RADIATIONCOOLING's -2 K/day."""
+        if heating_rate is None:
+            self._check(self.lib.vvm_dev_set_radiation(self.h, None, int(cooling)), "set_radiation")
+            return
+        r = np.ascontiguousarray(heating_rate, dtype=np.float64)
+        assert r.shape == (self.cfg.nx, self.cfg.nz), r.shape
+        self._check(self.lib.vvm_dev_set_radiation(self.h, r.ctypes.data_as(PTR), int(cooling)), "set_radiation")
 
     def collect(self, capacity: int) -> None:
         """Record training samples of the learned guess (after upload)."""

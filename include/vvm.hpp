@@ -22,6 +22,7 @@
 // (src/PoissonSolver.cpp:463-467).
 #pragma once
 
+#include <functional>
 #include <string>
 
 #include "vvm_dev.h"
@@ -112,6 +113,11 @@ class vvm {
     double **u = nullptr, **w = nullptr, **RKM = nullptr, **RKH = nullptr;
     double **evaporation = nullptr, **accretion = nullptr, **autoconversion = nullptr, **condensation = nullptr;
     double **init_th_forcing = nullptr;
+    // radiation heating rate [K/day] that AddForcing adds to thp
+    // (src/Declare.hpp:561, AddForcing.cpp:20); zero unless a radiation
+    // scheme fills it (solve_radiation below)
+    double **radiation_heating_rate = nullptr;
+    double *radiation_heating_ratecont = nullptr;
     double *zetapcont = nullptr, *zetacont = nullptr, *zetamcont = nullptr;
     double *thpcont = nullptr, *thcont = nullptr, *thmcont = nullptr;
     double *qvpcont = nullptr, *qvcont = nullptr, *qvmcont = nullptr;
@@ -149,6 +155,16 @@ class vvm {
     bool output_nc_same_file = false;  // OUTPUTNCSAMEFILE: append [t][x][z] slices to nc/<step/1200000>.nc
     bool output_microphysics = false;  // OUTPUTMICROPHYSICS: Kessler rates in the same-file output
     bool output_state_files = false;   // ... and state/state_<step>.bin (exact restart)
+    // Radiation coupling (src/Iteration.cpp:427-431, src/Rrtmgp.cpp): with a
+    // solve_radiation set, TimeMarching calls it after every step that is a
+    // multiple of radiation_every, with th, qv, qc, qr and zeta holding the
+    // values the reference's Radiation::solve_radiation reads there (the
+    // step's current time level, before nextTimeStep), and passes the
+    // radiation_heating_rate it leaves to the device for the next steps.
+    // radiation_cooling is RADIATIONCOOLING (-2 K/day on thp).
+    std::function<void(vvm &)> solve_radiation;
+    int radiation_every = 40;
+    bool radiation_cooling = false;
     long long solve_iters_total = 0;   // PCG iterations over the run
     vvm_host_view host_view();          // pointers into this model's arrays
     void sync_to_device();              // upload when dev_stale

@@ -203,6 +203,15 @@ int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out);
  * effect from the next step.  The coarse budget needs an uploaded state. */
 int vvm_dev_set_shape(vvm_dev_t *h, int diffusion_strip_waves, int mg_coarse_lds_kib);
 
+/* Host coupling of the radiation scheme (src/Rrtmgp.cpp, src/AddForcing.cpp):
+ * heating_rate is the reference's radiation_heating_rate, [nx][nz] in the
+ * reference's layout, K/day, which every later step adds to thp as
+ * AddForcing.cpp:20 does (thp += dt * rate / 86400); NULL removes it.
+ * cooling 1 adds RADIATIONCOOLING's -2 K/day (:21-23).  The host refreshes
+ * the rate as often as its radiation runs (every 40 steps in the reference,
+ * src/Iteration.cpp:427-431), from the state it downloads. */
+int vvm_dev_set_radiation(vvm_dev_t *h, const double *heating_rate, int cooling);
+
 /* Host-side helpers (no GPU use). */
 
 /* Fill the analytic base state and initial fields of one of the benchmark

@@ -61,6 +61,16 @@ def op(name: str, cfg, st, gsolve: int = GSOLVE_EXACT) -> None:
         _call(OPS[name], cfg, st)
 
 
+def set_radiation(rate, cooling: bool = False) -> None:
+    """AddForcing's radiation heating rate (K/day, (nx, nz), kept alive by
+    the caller) and RADIATIONCOOLING for later op/step calls; None clears."""
+    lib = load()
+    P = C.POINTER(C.c_double)
+    lib.vo_set_radiation.argtypes = [P, C.c_int]
+    lib.vo_set_radiation.restype = None
+    lib.vo_set_radiation(None if rate is None else rate.ctypes.data_as(P), int(cooling))
+
+
 def poisson_exact(cfg, rhou, rhow, b):
     import numpy as np
     lib = load()

@@ -339,9 +339,20 @@ int vo_pqr_pt(const vvm_dev_config *c, vvm_host_view *v) {
     return 0;
 }
 
-/* vvm::AddForcing (src/AddForcing.cpp:6-28, radiation term off; Q1LS/Q2LS
- * carry the configs' surface fluxes) followed by BoundaryProcess2D_all
- * (src/Iteration.cpp:283-285). */
+/* The radiation heating rate [K/day, [nx][nz]] and RADIATIONCOOLING of
+ * AddForcing, set by the caller (vo_set_radiation); NULL / 0 by default,
+ * which is what the benchmark configurations use. */
+static const double *vo_rad_rate = NULL;
+static int vo_rad_cooling = 0;
+void vo_set_radiation(const double *rate, int cooling) {
+    vo_rad_rate = rate;
+    vo_rad_cooling = cooling;
+}
+
+/* vvm::AddForcing (src/AddForcing.cpp:6-28; Q1LS/Q2LS carry the configs'
+ * surface fluxes) followed by BoundaryProcess2D_all (src/Iteration.cpp:
+ * 283-285).  The radiation term (:20) and RADIATIONCOOLING (:21-23) apply
+ * when vo_set_radiation gave them. */
 int vo_forcing_bc(const vvm_dev_config *c, vvm_host_view *v) {
     const int nx = c->nx, nz = c->nz;
     const double dt = c->dt;
@@ -351,6 +362,8 @@ int vo_forcing_bc(const vvm_dev_config *c, vvm_host_view *v) {
                 AT(v->thp, i, k) += dt * v->Q1LS[k];
                 AT(v->qvp, i, k) += dt * v->Q2LS[k];
             }
+            if (vo_rad_rate) AT(v->thp, i, k) += dt * vo_rad_rate[(size_t)i * nz + k] / 86400.;
+            if (vo_rad_cooling) AT(v->thp, i, k) += dt * (-2. / 86400.);
             if (v->status_for_adding_forcing) AT(v->thp, i, k) += dt * AT(v->init_th_forcing, i, k);
         }
     }

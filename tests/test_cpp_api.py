@@ -229,3 +229,85 @@ def test_same_file_output_appends_to_a_file_another_library_wrote(tmp_path):
         np.testing.assert_array_equal(th[2:, 2, 3], [400.0, 401.0, 402.0])
         np.testing.assert_array_equal(nc.variables["ubarTop"].data, [-5.0, -5.0, 10.0, 11.0, 12.0])
         np.testing.assert_array_equal(nc.variables["precip"].data[:, 5], [-5.0, -5.0, 0.0, -1.0, -2.0])
+
+
+RAD_PROG = r'''
+#include "vvm.hpp"
+#include <string>
+int main(int argc, char **argv) {
+    const std::string out = argv[1];
+    // c0 (as the driver builds it from the C0 keys): 128 x 64 interior at 200 m, dt 2 s, 7 steps
+    Config_VVM config(2., 200., 200., 128 * 200, 64 * 200, 14., 10000, out + "/", 0, 10., 10., 0.01, 1E-22,
+                      9.80665, 1003.5, 716.5, 287., 2.5E6, 1E5, 96500., 0., 1, 0., 2025, 3, 21, 12., 0., 0., 0.,
+                      0.);
+    config.tropical_forcing = 1;
+    vvm_case_params p{};
+    p.brunt = 0.01;
+    p.bubble_dT = 2.;
+    p.bubble_radius = 2000.;
+    p.bubble_zc = 2000.;
+    p.noise_kind = 1;
+    p.noise_amp = 0.01;
+    p.noise_ztop = 1000.;
+    p.seed = 1;
+    p.sfc_qv_ramp_z = 500.;
+    vvm model(config);
+    model.print_steps = false;
+    vvm::Init::InitCase(model, p);
+    vvm::PoissonSolver::InitPoissonMatrix(model);
+    vvm::Output::write_state(out + "/s0.bin", model);
+    // a stand-in radiation scheme: relaxation of theta and a moisture term,
+    // from the fields Radiation::solve_radiation reads
+    model.radiation_every = 3;
+    model.radiation_cooling = true;
+    model.solve_radiation = [](vvm &m) {
+        for (int i = 0; i < m.nx; i++)
+            for (int k = 0; k < m.nz; k++)
+                m.radiation_heating_rate[i][k] = -0.5 * (m.th[i][k] - 300.) + 2000. * m.qv[i][k];
+    };
+    vvm::Iteration::TimeMarching(model);
+    vvm::Output::write_state(out + "/final.bin", model);
+    return 0;
+}
+'''
+
+
+@pytest.mark.gpu
+def test_radiation_coupling_time_marching(tmp_path, oracle):
+    """TimeMarching with a host radiation scheme (Iteration.cpp:427-431):
+    called after steps 3 and 6 on the step's current time level, its
+    radiation_heating_rate entering AddForcing from the next step on, with
+    RADIATIONCOOLING.  The oracle runs the same 7 steps, recomputing the rate
+    from its own state at the same points."""
+    built()
+    src = tmp_path / "rad.cpp"
+    src.write_text(RAD_PROG)
+    exe = tmp_path / "rad"
+    root = os.path.dirname(PKG)
+    r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root}/include", str(src), "-o", str(exe), f"-L{PKG}",
+                        "-lvvm", f"-Wl,-rpath,{PKG}"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    run = tmp_path / "run"
+    run.mkdir()
+    print("radiation coupling: 7 steps of c0 through TimeMarching", flush=True)
+    r = subprocess.run([str(exe), str(run)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s0 = V.HostState.load(str(run / "s0.bin"))
+    got = V.HostState.load(str(run / "final.bin"))
+    assert got.step == 7
+    c = V.case("c0")
+    want = s0.copy()
+    rate = np.zeros((c.cfg.nx, c.cfg.nz))
+    try:
+        for s in range(1, 8):
+            oracle.set_radiation(rate, True)
+            oracle.step(c.cfg, want, 1)
+            if s % 3 == 0:  # the level before nextTimeStep is now the m level
+                rate = -0.5 * (want["thm"] - 300.) + 2000. * want["qvm"]
+    finally:
+        oracle.set_radiation(None, False)
+    assert_fields_close(got, want, ["zeta", "th", "qv", "qc", "qr", "u", "w"], 1e-9, where="TimeMarching with radiation")
+    # the hook's rate changed the run: without it the state differs
+    plain = s0.copy()
+    oracle.step(c.cfg, plain, 7)
+    assert np.abs(plain["th"] - got["th"]).max() > 1e-6

@@ -206,3 +206,33 @@ def test_tuner_keeps_state_and_parity(oracle):
         got = st.copy()
         dev.download(got)
     assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="tuned 1024x64 2 steps")
+
+
+@pytest.mark.parametrize("cooling", [False, True])
+def test_radiation_coupling_parity(oracle, cooling):
+    """A host-supplied radiation heating rate (the reference's
+    radiation_heating_rate, K/day, [i][k]) and RADIATIONCOOLING enter the
+    step as AddForcing does: the forcing operator to 1e-13 and two whole
+    steps to 1e-10 against the oracle with the same rate."""
+    c = V.case("c0", tropical_forcing=1)
+    st = perturbed_state(c, seed=43, step=2)
+    rate = np.random.default_rng(44).normal(0.0, 5.0, (c.cfg.nx, c.cfg.nz))
+    print(f"radiation coupling cooling={cooling}", flush=True)
+    want_op, want = st.copy(), st.copy()
+    try:
+        oracle.set_radiation(rate, cooling)
+        oracle.op("forcing_bc", c.cfg, want_op)
+        oracle.step(c.cfg, want, 2)
+    finally:
+        oracle.set_radiation(None, False)
+    with V.Device(c.cfg) as dev:
+        dev.set_radiation(rate, cooling)
+        got_op = gpu_run_op(dev, st, "forcing_bc")
+    assert_fields_close(got_op, want_op, ["thp", "qvp"], 1e-13, where="forcing with radiation")
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.set_radiation(rate, cooling)
+        dev.step(2)
+        got = st.copy()
+        dev.download(got)
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="c0 2 steps with radiation")

@@ -179,3 +179,28 @@ def test_init_base_state_closed_forms():
     d = st["th"][1:-1, 1:-1] - st["thb"][None, 1:-1]
     assert 1.9 < d.max() <= 2.0 + 0.01
     assert np.abs(d[:, z[1:-1] > 4100]).max() == 0.0
+
+
+@pytest.mark.parametrize("cooling", [False, True])
+def test_radiation_heating_term(oracle, cooling):
+    """AddForcing's radiation term and RADIATIONCOOLING (src/AddForcing.cpp:
+    20-23): against the same forcing without them, thp at every interior
+    point gains dt rate / 86400 (and dt (-2 / 86400)) and nothing else
+    changes; a dropped 1/86400, a sign or an [i][k] / [k][i] mix-up fails."""
+    c = V.case("tiny", tropical_forcing=1)
+    st = perturbed_state(c, seed=41, step=3)
+    nx, nz, dt = c.cfg.nx, c.cfg.nz, c.cfg.dt
+    rate = np.random.default_rng(42).normal(0.0, 5.0, (nx, nz))
+    rate[:, :nz // 2] += 3.0  # not symmetric under a transpose
+    base, rad = st.copy(), st.copy()
+    oracle.op("forcing_bc", c.cfg, base)
+    try:
+        oracle.set_radiation(rate, cooling)
+        oracle.op("forcing_bc", c.cfg, rad)
+    finally:
+        oracle.set_radiation(None, False)
+    want = dt * rate[1:-1, 1:-1] / 86400. + (dt * (-2. / 86400.) if cooling else 0.)
+    got = rad["thp"][1:-1, 1:-1] - base["thp"][1:-1, 1:-1]
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
+    for f in ("qvp", "qcp", "qrp", "zetap"):
+        np.testing.assert_array_equal(rad[f], base[f])
