@@ -667,10 +667,11 @@ static void free_pcg(PCG<T> &P) {
 void Poisson::destroy() {
     free_pcg(pcg64);
     if (have32) free_pcg(pcg32);
-    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)x64B, (void *)x32B, (void *)r64, (void *)gfeat,
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)x64B, (void *)x32B, (void *)bcg, (void *)bsc,
+                    (void *)r64, (void *)gfeat,
                     (void *)gsc, (void *)samples})
         if (p) hipFree(p);
-    f = xprev = xprev2 = r64 = x64B = nullptr;
+    f = xprev = xprev2 = r64 = x64B = bcg = bsc = nullptr;
     gfeat = samples = x32B = nullptr;
     bn_pending = 0;
     x_pending = 0;

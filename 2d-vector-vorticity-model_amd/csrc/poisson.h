@@ -66,6 +66,10 @@ class Poisson {
     SolveResult solve_fp64(double tol, int max_iter, hipStream_t st);
     // FP32 MG-PCG inside FP64 iterative refinement.
     SolveResult solve_mixed(double tol, int max_outer, int max_inner, double inner_tol, hipStream_t st);
+    // The reference GPU back-end's solver (k_bicgstab.hip): BiCGSTAB with a
+    // red-black ILU0 preconditioner; stops at ||b - A x||_2 <= tol
+    // (absolute) or <= tol ||b||_2.  Needs an even column count.
+    SolveResult solve_bicgstab(double tol, bool absolute, int max_iter, hipStream_t st);
 
     // Re-plan which multigrid levels the one-workgroup coarse kernel takes
     // (LDS budget in bytes, at most 64 KiB); a tuning knob, the
@@ -120,6 +124,7 @@ class Poisson {
     int x_pending = 0;
     bool corr_pending = false;
     double *x64B = nullptr;  // the other FP64 solution buffer (the correction writes it)
+    double *bcg = nullptr, *bsc = nullptr;  // BiCGSTAB vectors and scalars (allocated on first use)
     float *x32B = nullptr;   // the other FP32 solution buffer (zeroed while the correction reads x)
     void flush_pending(hipStream_t st);
     // r = f - L x (x with the pending guess or correction applied, and
@@ -131,6 +136,7 @@ class Poisson {
 // cal_u on the device (src/PoissonSolver.cpp:580-653): top-row potential
 // flow and the downward integration of u.
 bool cal_u_supported(const Grid &G);  // nx - 2 <= 8192
+bool bicgstab_supported(int N);      // red-black colouring: N even
 void launch_cal_u(const Grid &G, const StepPtrs &S, hipStream_t st);
 
 }  // namespace vvmd

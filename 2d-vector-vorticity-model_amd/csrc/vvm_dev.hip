@@ -252,7 +252,10 @@ int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss) {
     if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP : c.guess_mode, h->st);
     SolveResult R;
     const int maxit = c.max_iter > 0 ? c.max_iter : 200;
-    if (c.solver_mode == VVM_SOLVER_MIXED) {
+    if (c.solver_mode == VVM_SOLVER_BICGSTAB) {
+        const double tol = (c.stop_mode == VVM_STOP_PARITY) ? 1e-13 : c.tol_rel;
+        R = h->pois.solve_bicgstab(tol, c.stop_mode == VVM_STOP_ABS, maxit, h->st);
+    } else if (c.solver_mode == VVM_SOLVER_MIXED) {
         const double tol = (c.stop_mode == VVM_STOP_PARITY) ? 1e-13 : c.tol_rel;
         R = h->pois.solve_mixed(tol, 8, maxit, 1e-4, h->st);
     } else {
@@ -308,9 +311,16 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     G.Rd = cfg->Rd;
     G.Lv = cfg->Lv;
     G.P0 = cfg->P0;
-    if (!diffusion_supported(G) || !cal_u_supported(G)) {
+    if (!diffusion_supported(G) || !cal_u_supported(G) ||
+        (cfg->solver_mode == VVM_SOLVER_BICGSTAB && !bicgstab_supported(G.nx - 2))) {
         delete h;
         return VVM_ERR_SIZE;
+    }
+    if (cfg->solver_mode < VVM_SOLVER_FP64 || cfg->solver_mode > VVM_SOLVER_BICGSTAB ||
+        cfg->stop_mode < VVM_STOP_PARITY || cfg->stop_mode > VVM_STOP_ABS ||
+        (cfg->stop_mode == VVM_STOP_ABS && cfg->solver_mode != VVM_SOLVER_BICGSTAB)) {
+        delete h;
+        return VVM_ERR_ARG;
     }
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
         delete h;
@@ -760,7 +770,10 @@ static int poisson_common(vvm_dev *h, const double *d_b, double *d_x, vvm_solve_
     hipEventRecord(h->ev[6], h->st);
     SolveResult R;
     const int maxit = c.max_iter > 0 ? c.max_iter : 200;
-    if (c.solver_mode == VVM_SOLVER_MIXED) {
+    if (c.solver_mode == VVM_SOLVER_BICGSTAB) {
+        R = h->pois.solve_bicgstab(c.stop_mode == VVM_STOP_PARITY ? 1e-13 : c.tol_rel, c.stop_mode == VVM_STOP_ABS,
+                                   maxit, h->st);
+    } else if (c.solver_mode == VVM_SOLVER_MIXED) {
         R = h->pois.solve_mixed(c.stop_mode == VVM_STOP_PARITY ? 1e-13 : c.tol_rel, 8, maxit, 1e-4, h->st);
     } else {
         R = h->pois.solve_fp64(c.stop_mode == VVM_STOP_PARITY ? 0. : c.tol_rel, maxit, h->st);

@@ -19,7 +19,8 @@
 //   VVM_BRUNT VVM_BUBBLE_DT VVM_BUBBLE_RADIUS VVM_BUBBLE_ZC
 //   VVM_NOISE (none|uniform|gaussian) VVM_NOISE_AMP VVM_NOISE_ZTOP VVM_SEED
 //   VVM_SFC_HEAT_FLUX VVM_SFC_QV_RAMP VVM_SFC_QV_RAMP_Z
-//   VVM_SOLVER (fp64|mixed) VVM_GUESS (zero|prev|extrap) VVM_STOP (parity|rel)
+//   VVM_SOLVER (fp64|mixed|bicgstab) VVM_GUESS (zero|prev|extrap)
+//   VVM_STOP (parity|rel|abs; abs: bicgstab only, AMGX's absolute criterion)
 //   VVM_TOL_REL VVM_MAX_ITER VVM_GPU VVM_PRINT_STEPS (0|1)
 //   VVM_FINAL_STATE (path of a VVMSTATE file written after the run)
 //   VVM_OUTPUT_NC (1: <outputpath>/nc/<step>.nc on output steps, as the
@@ -117,12 +118,14 @@ int main(int argc, char **argv) {
                       K.d("VVM_MINUTE", 0.), K.d("VVM_SECOND", 0.), K.d("VVM_LON", 0.), K.d("VVM_LAT", 0.));
     config.gpu_id = K.i("VVM_GPU", 0);
     config.solver_mode =
-        pick(K.s("VVM_SOLVER", "fp64"), {{"fp64", VVM_SOLVER_FP64}, {"mixed", VVM_SOLVER_MIXED}}, "VVM_SOLVER");
+        pick(K.s("VVM_SOLVER", "fp64"),
+             {{"fp64", VVM_SOLVER_FP64}, {"mixed", VVM_SOLVER_MIXED}, {"bicgstab", VVM_SOLVER_BICGSTAB}}, "VVM_SOLVER");
     config.guess_mode = pick(K.s("VVM_GUESS", "zero"),
                              {{"zero", VVM_GUESS_ZERO}, {"prev", VVM_GUESS_PREV}, {"extrap", VVM_GUESS_EXTRAP}},
                              "VVM_GUESS");
     config.stop_mode =
-        pick(K.s("VVM_STOP", "parity"), {{"parity", VVM_STOP_PARITY}, {"rel", VVM_STOP_REL}}, "VVM_STOP");
+        pick(K.s("VVM_STOP", "parity"), {{"parity", VVM_STOP_PARITY}, {"rel", VVM_STOP_REL}, {"abs", VVM_STOP_ABS}},
+             "VVM_STOP");
     config.tol_rel = K.d("VVM_TOL_REL", 1e-10);
     config.max_iter = K.i("VVM_MAX_ITER", 2000);
     config.tropical_forcing = K.i("VVM_TROPICAL_FORCING", 0);
@@ -148,7 +151,9 @@ int main(int argc, char **argv) {
     model.output_microphysics = K.i("VVM_OUTPUT_MICROPHYSICS", 0) != 0;
     model.output_state_files = K.i("VVM_OUTPUT_STATE", 0) != 0;
     std::printf("nx=%d nz=%d dx=%g dz=%g dt=%g steps=%d solver=%s\n", model.nx, model.nz, model.dx, model.dz,
-                model.dt, int(model.TIMEEND / model.dt), config.solver_mode == VVM_SOLVER_MIXED ? "mixed" : "fp64");
+                model.dt, int(model.TIMEEND / model.dt), config.solver_mode == VVM_SOLVER_MIXED      ? "mixed"
+                : config.solver_mode == VVM_SOLVER_BICGSTAB ? "bicgstab"
+                                                            : "fp64");
     if (dry) return 0;
 
     if (K.s("VVM_INIT", "case") == "previous") {

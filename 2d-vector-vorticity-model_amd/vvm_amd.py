@@ -20,9 +20,9 @@ import numpy as np
 PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libvvm_amd.so")
 
-VVM_SOLVER_FP64, VVM_SOLVER_MIXED = 0, 1
+VVM_SOLVER_FP64, VVM_SOLVER_MIXED, VVM_SOLVER_BICGSTAB = 0, 1, 2
 VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED = 0, 1, 2, 3
-VVM_STOP_PARITY, VVM_STOP_REL = 0, 1
+VVM_STOP_PARITY, VVM_STOP_REL, VVM_STOP_ABS = 0, 1, 2
 KT = {"none": 0, "pzeta": 1, "thermo4": 2, "scale": 3, "kessler": 4, "turb": 5, "diffusion": 6, "mean": 7,
       "calu": 8, "solve": 9}
 

@@ -43,6 +43,12 @@ extern "C" {
 /* Poisson back-ends (north of the reference's single Eigen path). */
 #define VVM_SOLVER_FP64 0   /* FP64 multigrid-preconditioned CG            */
 #define VVM_SOLVER_MIXED 1  /* FP32 MG-PCG inside FP64 iterative refinement */
+/* The reference GPU back-end's solver (src/PoissonSolver_GPU_mpi.cpp:41:
+ * AMGX BiCGSTAB + ILU0, tolerance 1e-12, 10000 iterations): BiCGSTAB with a
+ * red-black ILU0 preconditioner, for comparison runs; with VVM_STOP_ABS and
+ * tol_rel = 1e-12 it stops as AMGX's default ABSOLUTE criterion does.
+ * Needs an even interior column count (nx - 2). */
+#define VVM_SOLVER_BICGSTAB 2
 
 /* Initial guess for the cal_w unknown x = rhow[k]*w[i][k]. */
 #define VVM_GUESS_ZERO 0     /* reference behaviour (Eigen x.setZero())     */
@@ -53,6 +59,7 @@ extern "C" {
 /* Stop rules of the elliptic solve. */
 #define VVM_STOP_PARITY 0  /* iterate until the residual stops decreasing   */
 #define VVM_STOP_REL 1     /* ||b - A x||_2 / ||b||_2 <= tol_rel            */
+#define VVM_STOP_ABS 2     /* ||b - A x||_2 <= tol_rel (BiCGSTAB only)       */
 
 /* Model configuration: the runtime members of the reference's Config_VVM
  * (src/Declare.hpp:27-70) that the timestep reads, plus back-end options. */

@@ -204,3 +204,81 @@ def test_radiation_heating_term(oracle, cooling):
     np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
     for f in ("qvp", "qcp", "qrp", "zetap"):
         np.testing.assert_array_equal(rad[f], base[f])
+
+
+def test_red_black_ilu0_formulas():
+    """The red-black ILU0 the BiCGSTAB mode applies (k_bicgstab.hip: a black
+    pass z_b = (r_b - sum a_bq r_q / d_q) / S_b, S_b = d_b - sum a_bq^2 / d_q,
+    then a red pass z_r = (r_r - sum a_rq z_q) / d_r) against a textbook
+    ILU(0) (IKJ elimination restricted to the pattern of A) of the same
+    matrix in red-black order, on the tiny grid's operator."""
+    c = V.case("tiny")
+    st = V.init_state(c)
+    N, M = c.cfg.nx - 2, c.cfg.nz - 3
+    assert N % 2 == 0
+    rhou, rhow = st["rhou"], st["rhow"]
+    # the device operator L of the unknown grid: rows j = 1..M, periodic i
+    cx = np.zeros(M + 2)
+    cu = np.zeros(M + 1)
+    cx[1:M + 1] = 1.0 / rhow[2:M + 2]
+    cu[0:M + 1] = 1.0 / rhou[1:M + 2]
+    idx = lambda i, j: (j - 1) * N + i  # noqa: E731
+    n = N * M
+    A = np.zeros((n, n))
+    for j in range(1, M + 1):
+        for i in range(N):
+            p = idx(i, j)
+            A[p, p] = 2 * cx[j] + cu[j] + cu[j - 1]
+            A[p, idx((i - 1) % N, j)] -= cx[j]
+            A[p, idx((i + 1) % N, j)] -= cx[j]
+            if j < M:
+                A[p, idx(i, j + 1)] -= cu[j]
+            if j > 1:
+                A[p, idx(i, j - 1)] -= cu[j - 1]
+    red = [idx(i, j) for j in range(1, M + 1) for i in range(N) if (i + j) % 2 == 0]
+    black = [idx(i, j) for j in range(1, M + 1) for i in range(N) if (i + j) % 2 == 1]
+    order = red + black
+    Ap = A[np.ix_(order, order)]
+    # textbook ILU(0): IKJ Gaussian elimination, fill outside the pattern dropped
+    LU = Ap.copy()
+    pat = Ap != 0
+    for i in range(1, n):
+        for k in range(i):
+            if not pat[i, k]:
+                continue
+            LU[i, k] /= LU[k, k]
+            for jj in range(k + 1, n):
+                if pat[i, jj]:
+                    LU[i, jj] -= LU[i, k] * LU[k, jj]
+    Lm = np.tril(LU, -1) + np.eye(n)
+    Um = np.triu(LU)
+    r = np.random.default_rng(7).normal(size=n)
+    want = np.zeros(n)
+    want[order] = np.linalg.solve(Um, np.linalg.solve(Lm, r[order]))
+    # the kernels' two passes
+    d = lambda j: 2 * cx[j] + cu[j] + cu[j - 1]  # noqa: E731
+    z = np.zeros(n)
+    for j in range(1, M + 1):
+        for i in range(N):
+            if (i + j) % 2 == 0:
+                continue
+            S = d(j) - 2 * cx[j] ** 2 / d(j)
+            acc = -cx[j] * (r[idx((i - 1) % N, j)] + r[idx((i + 1) % N, j)]) / d(j)
+            if j < M:
+                S -= cu[j] ** 2 / d(j + 1)
+                acc += -cu[j] * r[idx(i, j + 1)] / d(j + 1)
+            if j > 1:
+                S -= cu[j - 1] ** 2 / d(j - 1)
+                acc += -cu[j - 1] * r[idx(i, j - 1)] / d(j - 1)
+            z[idx(i, j)] = (r[idx(i, j)] - acc) / S
+    for j in range(1, M + 1):
+        for i in range(N):
+            if (i + j) % 2 == 1:
+                continue
+            acc = -cx[j] * (z[idx((i - 1) % N, j)] + z[idx((i + 1) % N, j)])
+            if j < M:
+                acc += -cu[j] * z[idx(i, j + 1)]
+            if j > 1:
+                acc += -cu[j - 1] * z[idx(i, j - 1)]
+            z[idx(i, j)] = (r[idx(i, j)] - acc) / d(j)
+    np.testing.assert_allclose(z, want, rtol=1e-12, atol=1e-14)
