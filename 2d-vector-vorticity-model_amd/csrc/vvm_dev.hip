@@ -63,6 +63,12 @@ struct vvm_dev {
     // qv, qc, qr, the microphysics rates and the negval reduction slots and
     // scalars, none of which B touches
     hipStream_t side = nullptr;
+    // host microphysics in place of Kessler (vvm_dev_set_host_microphysics):
+    // the callback, and the host copy of the state it is handed each step
+    vvm_host_microphysics_fn mp_fn = nullptr;
+    void *mp_user = nullptr;
+    std::vector<std::vector<double>> mp_store;
+    vvm_host_view mp_view{};
     // host-supplied radiation heating (vvm_dev_set_radiation)
     double *rad = nullptr;
     bool rad_on = false;
@@ -552,6 +558,76 @@ int vvm_dev_download(vvm_dev_t *h, vvm_host_view *v) {
 
 // Host side of the random theta kick (src/Iteration.cpp:276-282): the
 // reference draws it on the host each step; so does this path, then copies it.
+// Phase C on the host: the step's fields into the handle's host view (the
+// reference's [i][k] layout), the caller's microphysics, and the fields it
+// may change (thp, qvp, qcp, qrp, the four process rates) back; then the
+// halos of the p fields (src/Iteration.cpp:404).
+static int host_microphysics(vvm_dev *h, const StepPtrs &S) {
+    const Grid &G = h->G;
+    const size_t n2 = size_t(G.nx) * G.nz;
+    vvm_host_view &v = h->mp_view;
+    if (h->mp_store.empty()) {
+        auto take = [&](size_t n) {
+            h->mp_store.emplace_back(n, 0.);
+            return h->mp_store.back().data();
+        };
+        h->mp_store.reserve(64);
+        double **f2[] = {&v.zeta, &v.zetam, &v.zetap, &v.th, &v.thm, &v.thp, &v.qv, &v.qvm, &v.qvp,
+                         &v.qc, &v.qcm, &v.qcp, &v.qr, &v.qrm, &v.qrp, &v.u, &v.w, &v.RKM, &v.RKH,
+                         &v.init_th_forcing, &v.evaporation, &v.accretion, &v.autoconversion, &v.condensation};
+        for (double **p : f2) *p = take(n2);
+        double **f1x[] = {&v.precip, &v.xi, &v.uxi};
+        for (double **p : f1x) *p = take(G.nx);
+        double **f1z[] = {&v.rhou, &v.rhow, &v.pib, &v.pb, &v.thb, &v.thbm, &v.thb_init, &v.thb_zeta, &v.thvb,
+                          &v.thvbm, &v.qvb, &v.qvb0, &v.qvsb, &v.z, &v.z_zeta, &v.lambda2, &v.Q1LS, &v.Q2LS};
+        for (double **p : f1z) *p = take(G.nz);
+    }
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    double *trip[NPROG][3] = {{v.zeta, v.zetam, v.zetap}, {v.th, v.thm, v.thp}, {v.qv, v.qvm, v.qvp},
+                              {v.qc, v.qcm, v.qcp},       {v.qr, v.qrm, v.qrp}};
+    for (int f = 0; f < NPROG; f++) {
+        const double *src[3] = {S.now[f], S.past[f], S.fut[f]};
+        for (int s = 0; s < 3; s++) {
+            int rc = get2d(h, src[s], trip[f][s]);
+            if (rc) return rc;
+        }
+    }
+    const double *d2[] = {S.u, S.w, S.RKM, S.RKH, S.forcing, S.evap, S.accr, S.autoc, S.cond};
+    double *h2[] = {v.u, v.w, v.RKM, v.RKH, v.init_th_forcing, v.evaporation, v.accretion, v.autoconversion,
+                    v.condensation};
+    for (int q = 0; q < 9; q++) {
+        int rc = get2d(h, d2[q], h2[q]);
+        if (rc) return rc;
+    }
+    HIPCHK(h, hipMemcpy(v.precip, S.precip, G.nx * sizeof(double), hipMemcpyDeviceToHost));
+    const double *p1[] = {h->P.rhou, h->P.rhow, h->P.pib, h->P.pb, h->P.thb, h->P.thbm, h->P.thb_init,
+                          h->P.thb_zeta, h->P.thvb, h->P.thvbm, h->P.qvb, h->P.qvb0, h->P.qvsb, h->P.z,
+                          h->P.z_zeta, h->P.lambda2, h->P.Q1LS, h->P.Q2LS};
+    double *h1[] = {v.rhou, v.rhow, v.pib, v.pb, v.thb, v.thbm, v.thb_init, v.thb_zeta, v.thvb,
+                    v.thvbm, v.qvb, v.qvb0, v.qvsb, v.z, v.z_zeta, v.lambda2, v.Q1LS, v.Q2LS};
+    for (int q = 0; q < 18; q++) HIPCHK(h, hipMemcpy(h1[q], p1[q], G.nz * sizeof(double), hipMemcpyDeviceToHost));
+    v.step = h->step;
+    v.status_for_adding_forcing = h->status_for_adding_forcing;
+    const int urc = h->mp_fn(h->mp_user, &v);
+    if (urc != 0) return fail(h, VVM_ERR_STATE, "host microphysics returned " + std::to_string(urc));
+    const double *back[] = {v.thp, v.qvp, v.qcp, v.qrp, v.evaporation, v.accretion, v.autoconversion,
+                            v.condensation};
+    double *dst[] = {S.fut[F_TH], S.fut[F_QV], S.fut[F_QC], S.fut[F_QR], S.evap, S.accr, S.autoc, S.cond};
+    for (int q = 0; q < 8; q++) {
+        int rc = put2d(h, dst[q], back[q]);
+        if (rc) return rc;
+    }
+    launch_bc_all_prog(G, S, h->st);
+    return VVM_OK;
+}
+
+int vvm_dev_set_host_microphysics(vvm_dev_t *h, vvm_host_microphysics_fn fn, void *user) {
+    if (!h) return VVM_ERR_ARG;
+    h->mp_fn = fn;
+    h->mp_user = user;
+    return VVM_OK;
+}
+
 static int upload_random_kick(vvm_dev *h) {
     const size_t n = size_t(h->G.nx) * h->G.nz;
     std::vector<double> host(n, 0.);
@@ -606,18 +682,21 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         if (last) hipEventRecord(h->ev[1], h->st);
         // C: Kessler + NegativeValueProcess x3 + halos (:303-311, 404), on
         // the side stream while B runs; the main stream waits for it before D
+        // (with host microphysics, C runs after B on the host instead)
         hipEventRecord(h->ev_fork, h->st);
         hipStreamWaitEvent(h->side, h->ev_fork, 0);
-        if (last) hipEventRecord(h->ev_mp[0], h->side);
-        {
-            KTimer t(h, VVM_KT_KESSLER, h->side);
-            launch_kessler(G, S, h->red.negval, h->side);
+        if (!h->mp_fn) {
+            if (last) hipEventRecord(h->ev_mp[0], h->side);
+            {
+                KTimer t(h, VVM_KT_KESSLER, h->side);
+                launch_kessler(G, S, h->red.negval, h->side);
+            }
+            {
+                KTimer t(h, VVM_KT_SCALE, h->side);
+                launch_scale_forcing(G, S, false, false, false, h->side);
+            }
+            if (last) hipEventRecord(h->ev_mp[1], h->side);
         }
-        {
-            KTimer t(h, VVM_KT_SCALE, h->side);
-            launch_scale_forcing(G, S, false, false, false, h->side);
-        }
-        if (last) hipEventRecord(h->ev_mp[1], h->side);
         hipEventRecord(h->ev_join, h->side);
         // B: pubarTop_pt, cal_w, cal_u (:292-296)
         launch_pubartop(G, S, h->step, h->st);
@@ -641,6 +720,12 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         }
         if (last) hipEventRecord(h->ev[2], h->st);
         hipStreamWaitEvent(h->st, h->ev_join, 0);
+        if (h->mp_fn) {
+            if (last) hipEventRecord(h->ev_mp[0], h->st);
+            const int mrc = host_microphysics(h, S);
+            if (mrc) return mrc;
+            if (last) hipEventRecord(h->ev_mp[1], h->st);
+        }
         if (last) hipEventRecord(h->ev[3], h->st);
         // D: RKM_RKH with Mparam/Hparam, nudging (:408-417); the first
         // updateMean (:408) is overwritten by nextTimeStep's before any read.

@@ -772,6 +772,14 @@ void vvm::Iteration::TimeMarching(vvm &model) {
     const int nmax = int(model.TIMEEND / model.dt);
     ensure_device(model);
     const bool radiation = static_cast<bool>(model.solve_radiation) && model.radiation_every > 0;
+    if (model.host_microphysics) {
+        auto tramp = [](void *user, vvm_host_view *view) -> int {
+            return static_cast<vvm *>(user)->host_microphysics(*view);
+        };
+        check(model, vvm_dev_set_host_microphysics(model.dev, tramp, &model), "vvm_dev_set_host_microphysics");
+    } else {
+        check(model, vvm_dev_set_host_microphysics(model.dev, nullptr, nullptr), "vvm_dev_set_host_microphysics");
+    }
     auto push_radiation = [&]() {
         const int rc = vvm_dev_set_radiation(model.dev, model.radiation_heating_ratecont, model.radiation_cooling ? 1 : 0);
         check(model, rc, "vvm_dev_set_radiation");

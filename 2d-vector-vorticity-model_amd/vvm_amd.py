@@ -64,6 +64,10 @@ class VvmHostView(C.Structure):
                    ("status_for_adding_forcing", C.c_int32)])
 
 
+# int fn(void *user, vvm_host_view *view): a host microphysics scheme
+HOST_MP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(VvmHostView))
+
+
 class VvmStepStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("solve_iters_total", C.c_int32), ("solve_iters_last", C.c_int32),
                 ("refine_last", C.c_int32), ("not_converged", C.c_int32), ("rel_residual_last", C.c_double),
@@ -166,6 +170,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_ens_step": (C.c_int, [C.c_void_p, C.c_int, P(VvmEnsStats)]),
         "vvm_dev_set_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
         "vvm_dev_set_radiation": (C.c_int, [C.c_void_p, PTR, C.c_int]),
+        "vvm_dev_set_host_microphysics": (C.c_int, [C.c_void_p, HOST_MP_FN, C.c_void_p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name, None)
@@ -444,6 +449,18 @@ class Device:
 
     def set_shape(self, cw: int = 0, coarse_kib: int = 0) -> None:
         self._check(self.lib.vvm_dev_set_shape(self.h, cw, coarse_kib), "set_shape")
+
+    def set_host_microphysics(self, fn) -> None:
+        """Run fn(view) (view: a ctypes VvmHostView pointer into host arrays in
+        the reference's [i][k] layout) in place of the device Kessler each
+        step; it may change thp, qvp, qcp, qrp and the process rates and
+        returns 0.  None restores the device Kessler."""
+        if fn is None:
+            self._mp = None
+            self._check(self.lib.vvm_dev_set_host_microphysics(self.h, HOST_MP_FN(), None), "set_host_microphysics")
+            return
+        self._mp = HOST_MP_FN(lambda user, view: int(fn(view)))  # kept alive with the handle
+        self._check(self.lib.vvm_dev_set_host_microphysics(self.h, self._mp, None), "set_host_microphysics")
 
     def set_radiation(self, heating_rate: np.ndarray | None, cooling: bool = False) -> None:
         """The radiation heating rate [K/day], (nx, nz) in the reference's

@@ -165,6 +165,11 @@ class vvm {
     std::function<void(vvm &)> solve_radiation;
     int radiation_every = 40;
     bool radiation_cooling = false;
+    // Column microphysics on the host in place of Kessler (the reference's
+    // P3 call, src/Iteration.cpp:314-402): TimeMarching hands it each
+    // step's fields in the [i][k] layout (see vvm_dev_set_host_microphysics)
+    // and takes back thp, qvp, qcp, qrp and the process rates; it returns 0.
+    std::function<int(vvm_host_view &)> host_microphysics;
     long long solve_iters_total = 0;   // PCG iterations over the run
     vvm_host_view host_view();          // pointers into this model's arrays
     void sync_to_device();              // upload when dev_stale

@@ -219,6 +219,19 @@ int vvm_dev_set_shape(vvm_dev_t *h, int diffusion_strip_waves, int mg_coarse_lds
  * src/Iteration.cpp:427-431), from the state it downloads. */
 int vvm_dev_set_radiation(vvm_dev_t *h, const double *heating_rate, int cooling);
 
+/* Host coupling of a column microphysics scheme in place of Kessler (the
+ * reference's P3 call, src/Iteration.cpp:314-402, which runs on the host
+ * over [i][k] columns): each step, after cal_u, the handle downloads the
+ * step's fields into a host view in the reference's [i][k] layout (every
+ * time level, u, w, RKM, RKH, the process rates, precip and the profiles;
+ * no AB2 histories), calls fn(user, view), and uploads thp, qvp, qcp, qrp
+ * and the four process rates back, then applies the halos (:404).  The
+ * scheme replaces Kessler and its NegativeValueProcess; fn returns 0, or
+ * a non-zero code that stops the step with VVM_ERR_STATE.  NULL restores
+ * the device Kessler. */
+typedef int (*vvm_host_microphysics_fn)(void *user, vvm_host_view *view);
+int vvm_dev_set_host_microphysics(vvm_dev_t *h, vvm_host_microphysics_fn fn, void *user);
+
 /* Host-side helpers (no GPU use). */
 
 /* Fill the analytic base state and initial fields of one of the benchmark

@@ -290,3 +290,35 @@ def test_bicgstab_needs_even_columns():
     c = V.grid_case(127, 30, dx=100.0, dt=1.0, solver_mode=V.VVM_SOLVER_BICGSTAB)
     with pytest.raises(Exception):
         V.Device(c.cfg)
+
+
+def test_host_microphysics_coupling(oracle):
+    """A host column-microphysics scheme in place of the device Kessler (the
+    P3 coupling point, src/Iteration.cpp:314-402): the oracle's own Kessler
+    pass, called through vvm_dev_set_host_microphysics on the host view of
+    each step, gives the oracle's two steps to the 1e-10 bar, and is called
+    once a step with that step's number."""
+    import ctypes as C
+    c = V.case("c0")
+    st = V.init_state(c)
+    want = st.copy()
+    oracle.step(c.cfg, want, 2)
+    lib = oracle.load()
+    lib.vo_microphysics.argtypes = [C.POINTER(V.VvmDevConfig), C.POINTER(V.VvmHostView)]
+    lib.vo_microphysics.restype = C.c_int
+    cfg = c.cfg
+    calls = []
+
+    def kessler_on_host(view):
+        calls.append(view.contents.step)
+        return lib.vo_microphysics(C.byref(cfg), view)
+
+    print("host microphysics: c0, 2 steps", flush=True)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.set_host_microphysics(kessler_on_host)
+        dev.step(2)
+        got = st.copy()
+        dev.download(got)
+    assert calls == [st.step + 1, st.step + 2], calls
+    assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="c0 2 steps, host microphysics")
