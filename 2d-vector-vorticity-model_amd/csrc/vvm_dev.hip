@@ -63,6 +63,7 @@ struct vvm_dev {
     // qv, qc, qr, the microphysics rates and the negval reduction slots and
     // scalars, none of which B touches
     hipStream_t side = nullptr;
+    bool overlap_c = true;  // off for ensemble members (dev_set_phase_overlap)
     // host microphysics in place of Kessler (vvm_dev_set_host_microphysics):
     // the callback, and the host copy of the state it is handed each step
     vvm_host_microphysics_fn mp_fn = nullptr;
@@ -373,8 +374,11 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     for (auto &e : h->ev_mp) ok = ok && hipEventCreate(&e) == hipSuccess;
     ok = ok && hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) == hipSuccess &&
-         hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) == hipSuccess;
+         hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) == hipSuccess;
+    // the second stream is created on the first step that uses it: HIP hands
+    // streams to its few hardware queues in creation order, and a stream
+    // created here but never used (ensemble members) would leave the members'
+    // main streams on every other queue
     if (!ok) {
         vvm_dev_destroy(h);
         return VVM_ERR_NOMEM;
@@ -683,21 +687,27 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         // C: Kessler + NegativeValueProcess x3 + halos (:303-311, 404), on
         // the side stream while B runs; the main stream waits for it before D
         // (with host microphysics, C runs after B on the host instead)
-        hipEventRecord(h->ev_fork, h->st);
-        hipStreamWaitEvent(h->side, h->ev_fork, 0);
-        if (!h->mp_fn) {
-            if (last) hipEventRecord(h->ev_mp[0], h->side);
-            {
-                KTimer t(h, VVM_KT_KESSLER, h->side);
-                launch_kessler(G, S, h->red.negval, h->side);
-            }
-            {
-                KTimer t(h, VVM_KT_SCALE, h->side);
-                launch_scale_forcing(G, S, false, false, false, h->side);
-            }
-            if (last) hipEventRecord(h->ev_mp[1], h->side);
+        if (h->overlap_c && !h->mp_fn && !h->side)
+            HIPCHK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+        const bool overlap = h->overlap_c && !h->mp_fn;
+        hipStream_t cs = overlap ? h->side : h->st;
+        if (overlap) {
+            hipEventRecord(h->ev_fork, h->st);
+            hipStreamWaitEvent(h->side, h->ev_fork, 0);
         }
-        hipEventRecord(h->ev_join, h->side);
+        if (!h->mp_fn) {
+            if (last) hipEventRecord(h->ev_mp[0], cs);
+            {
+                KTimer t(h, VVM_KT_KESSLER, cs);
+                launch_kessler(G, S, h->red.negval, cs);
+            }
+            {
+                KTimer t(h, VVM_KT_SCALE, cs);
+                launch_scale_forcing(G, S, false, false, false, cs);
+            }
+            if (last) hipEventRecord(h->ev_mp[1], cs);
+        }
+        if (overlap) hipEventRecord(h->ev_join, h->side);
         // B: pubarTop_pt, cal_w, cal_u (:292-296)
         launch_pubartop(G, S, h->step, h->st);
         vvm_solve_stats ss{};
@@ -719,7 +729,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             launch_cal_u(G, S, h->st);
         }
         if (last) hipEventRecord(h->ev[2], h->st);
-        hipStreamWaitEvent(h->st, h->ev_join, 0);
+        if (overlap) hipStreamWaitEvent(h->st, h->ev_join, 0);
         if (h->mp_fn) {
             if (last) hipEventRecord(h->ev_mp[0], h->st);
             const int mrc = host_microphysics(h, S);
@@ -1089,6 +1099,12 @@ int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out) {
 }
 
 }  // extern "C"
+
+namespace vvmd {
+void dev_set_phase_overlap(vvm_dev_t *h, bool on) {
+    if (h) h->overlap_c = on;
+}
+}  // namespace vvmd
 
 namespace vvmd {
 int dev_samples(vvm_dev_t *h, const float **buf, int *N, int *M, int *count, int *device) {

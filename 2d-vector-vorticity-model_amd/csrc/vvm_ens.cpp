@@ -10,6 +10,11 @@
 #include <thread>
 #include <vector>
 
+namespace vvmd {
+// (vvm_dev.hip) members run their microphysics phase on their main stream
+void dev_set_phase_overlap(vvm_dev_t *h, bool on);
+}  // namespace vvmd
+
 struct vvm_ens {
     int device = 0, threads = 1;
     std::vector<vvm_dev_t *> dev;
@@ -66,6 +71,7 @@ int vvm_ens_create(const vvm_ens_member *m, int n, int device, int threads, vvm_
         int rc = vvm_init_case(&m[i].cfg, &m[i].params, &hs.v);
         vvm_dev_t *h = nullptr;
         if (rc == VVM_OK) rc = vvm_dev_create(&m[i].cfg, device, &h);
+        if (rc == VVM_OK) vvmd::dev_set_phase_overlap(h, false);
         if (rc == VVM_OK) rc = vvm_dev_upload(h, &hs.v);
         if (rc < 0) {
             const std::string msg = "member " + std::to_string(i) + ": " + (h ? vvm_dev_last_error(h) : "create");

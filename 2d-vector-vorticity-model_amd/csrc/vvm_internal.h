@@ -75,6 +75,11 @@ void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool t
                           hipStream_t st);
 void launch_bc_uw(const Grid &G, const StepPtrs &S, hipStream_t st);
 void launch_bc_all_prog(const Grid &G, const StepPtrs &S, hipStream_t st);
+// Whether a handle runs the microphysics phase on its second stream beside
+// the elliptic solve (default on).  The ensemble turns it off: with one
+// stream pair per member on a few hardware queues, the cross-stream waits
+// block other members' work queued behind them.
+void dev_set_phase_overlap(vvm_dev_t *h, bool on);
 void launch_pubartop(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
 
 // Physics (k_physics.hip)
