@@ -311,3 +311,69 @@ def test_radiation_coupling_time_marching(tmp_path, oracle):
     plain = s0.copy()
     oracle.step(c.cfg, plain, 7)
     assert np.abs(plain["th"] - got["th"]).max() > 1e-6
+
+
+MP_PROG = r'''
+#include "vvm.hpp"
+#include <string>
+extern "C" int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v);  // the oracle's Kessler pass
+int main(int argc, char **argv) {
+    const std::string out = argv[1];
+    Config_VVM config(2., 200., 200., 128 * 200, 64 * 200, 6., 10000, out + "/", 0, 10., 10., 0.01, 1E-22,
+                      9.80665, 1003.5, 716.5, 287., 2.5E6, 1E5, 96500., 0., 1, 0., 2025, 3, 21, 12., 0., 0., 0.,
+                      0.);
+    config.tropical_forcing = 1;
+    vvm_case_params p{};
+    p.brunt = 0.01;
+    p.bubble_dT = 2.;
+    p.bubble_radius = 2000.;
+    p.bubble_zc = 2000.;
+    p.noise_kind = 1;
+    p.noise_amp = 0.01;
+    p.noise_ztop = 1000.;
+    p.seed = 1;
+    p.sfc_qv_ramp_z = 500.;
+    vvm model(config);
+    model.print_steps = false;
+    vvm::Init::InitCase(model, p);
+    vvm::PoissonSolver::InitPoissonMatrix(model);
+    vvm::Output::write_state(out + "/s0.bin", model);
+    int calls = 0;
+    model.host_microphysics = [&](vvm_host_view &v) {
+        calls++;
+        return vo_microphysics(&model.dev_config, &v);
+    };
+    vvm::Iteration::TimeMarching(model);
+    vvm::Output::write_state(out + "/final.bin", model);
+    return calls == 3 ? 0 : 3;
+}
+'''
+
+
+@pytest.mark.gpu
+def test_host_microphysics_time_marching(tmp_path, oracle):
+    """TimeMarching with model.host_microphysics (P3's call point): the
+    oracle's Kessler pass, linked into the program and called once a step
+    on the host view, gives the oracle's own three steps."""
+    built()
+    src = tmp_path / "mp.cpp"
+    src.write_text(MP_PROG)
+    exe = tmp_path / "mp"
+    root = os.path.dirname(PKG)
+    odir = os.path.join(root, "oracle")
+    r = subprocess.run(["g++", "-std=c++17", "-O1", f"-I{root}/include", str(src), "-o", str(exe), f"-L{PKG}",
+                        "-lvvm", f"-Wl,-rpath,{PKG}", f"-L{odir}", "-loracle", f"-Wl,-rpath,{odir}"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    run = tmp_path / "run"
+    run.mkdir()
+    print("host microphysics: 3 steps of c0 through TimeMarching", flush=True)
+    r = subprocess.run([str(exe), str(run)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    s0 = V.HostState.load(str(run / "s0.bin"))
+    got = V.HostState.load(str(run / "final.bin"))
+    want = s0.copy()
+    oracle.step(V.case("c0").cfg, want, 3)
+    assert got.step == 3
+    assert_fields_close(got, want, ["zeta", "th", "qv", "qc", "qr", "u", "w"], 1e-9,
+                        where="TimeMarching with host microphysics")
