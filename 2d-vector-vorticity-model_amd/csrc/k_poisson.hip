@@ -484,6 +484,22 @@ void free_hier(MGHier<T> &H) {
     H.nlev = 0;
 }
 
+template <typename T>
+bool pcg_fused(const PCG<T> &P, const RedSlot &rs, const double *hpart_dev) {
+    return hpart_dev && mg_cg_fusable(P.mg, rs.max_blocks);
+}
+
+// The first V-cycle of pcg_loop (on P.r), which a caller may queue before it
+// has read the initial residual, so that the GPU does not wait for the host;
+// returns what pcg_loop's n_rz_pre takes.
+template <typename T>
+int pcg_first_vcycle(PCG<T> &P, const RedSlot &rs, const double *hpart_dev, hipStream_t st) {
+    const bool fused = pcg_fused(P, rs, hpart_dev);
+    // fused: r.z and w.z partials; otherwise r.z into sc[P_RZ]
+    const int n = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st, false, fused ? P.w : nullptr);
+    return fused ? n : 0;  // unfused: the first pq kernel takes beta = 0 and needs no partials
+}
+
 // Preconditioned CG iterations on P (x, r and sc[P_RNORM2] set up by the
 // caller; rn_start = that ||w r||^2) until ||w r||^2 <= target or max_iter.
 // The convergence test lags one V-cycle behind the GPU so the stream never
@@ -503,16 +519,14 @@ void free_hier(MGHier<T> &H) {
 template <typename T>
 int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedSlot &rs, double *hs,
              double *hs_dev, const double *hpart, double *hpart_dev, hipEvent_t ev, hipStream_t st,
-             double rn_start, double *rn2, bool *conv) {
-    const bool fused = hpart_dev && mg_cg_fusable(P.mg, rs.max_blocks);
+             double rn_start, double *rn2, bool *conv, int n_rz_pre = -1) {
+    const bool fused = pcg_fused(P, rs, hpart_dev);
     const int nb = cg_blocks(P.N, P.M);
     const dim3 b(LX, LY);
     // p.q partials in the last row of the reduction slots, which the update
     // kernel reads while it writes its own partials to row 0
     const RedSlot rs_pq{rs.partial + size_t(7) * rs.max_blocks, rs.max_blocks};
-    // fused: r.z and w.z partials; otherwise r.z into sc[P_RZ]
-    int n_rz = mg_vcycle(P.mg, P.r, P.zA, P.zB, rs, P.sc, true, st, false, fused ? P.w : nullptr);
-    if (!fused) n_rz = 0;  // the first pq kernel takes beta = 0 and needs no partials
+    int n_rz = n_rz_pre >= 0 ? n_rz_pre : pcg_first_vcycle(P, rs, hpart_dev, st);
     int it = 0;
     double best = 1e300;
     int stall = 0;
@@ -652,6 +666,7 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
         return VVM_ERR_NOMEM;
     if (hipHostGetDevicePointer(reinterpret_cast<void **>(&hpart_dev), hpart, 0) != hipSuccess) return VVM_ERR_HIP;
     if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return VVM_ERR_HIP;
+    if (hipEventCreateWithFlags(&ev_res, hipEventDisableTiming) != hipSuccess) return VVM_ERR_HIP;
     return VVM_OK;
 }
 
@@ -686,6 +701,8 @@ void Poisson::destroy() {
     if (hpart) hipHostFree(hpart);
     hpart = hpart_dev = nullptr;
     if (ev) hipEventDestroy(ev);
+    if (ev_res) hipEventDestroy(ev_res);
+    ev_res = nullptr;
     ev = nullptr;
 }
 
@@ -798,7 +815,11 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     SolveResult R;
     PCG<double> &P = pcg64;
     first_residual(P.r, false, st);
-    hipStreamSynchronize(st);
+    // the first V-cycle goes in before the host reads the residual (a guess
+    // that already meets the stop rule wastes it, which is rare)
+    hipEventRecord(ev_res, st);
+    const int n_rz_pre = pcg_first_vcycle(P, red, hpart_dev, st);
+    hipEventSynchronize(ev_res);
     const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
     if (bn2 == 0.) {
         hipMemsetAsync(P.x, 0, size_t(M + 2) * N * sizeof(double), st);
@@ -816,7 +837,8 @@ SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     }
     double last = rn2;
     bool conv = false;
-    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, rn2, &last, &conv);
+    R.iters = pcg_loop(P, target, max_iter, parity, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, rn2, &last,
+                       &conv, n_rz_pre);
     R.rel = std::sqrt(last / bn2);
     R.converged = conv;
     return R;
@@ -834,7 +856,12 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
     for (int outer = 0; outer <= max_outer; outer++) {
         // r64, and its FP32 copy for the correction solve (unused if converged)
         first_residual(r64, true, st);
-        hipStreamSynchronize(st);
+        // on the first sweep the inner solve's first V-cycle goes in before
+        // the host reads the residual (the guess almost never meets the stop
+        // rule); on later sweeps the residual is usually the final check
+        hipEventRecord(ev_res, st);
+        const int n_rz_pre = (outer == 0) ? pcg_first_vcycle(F, red, hpart_dev, st) : -1;
+        hipEventSynchronize(ev_res);
         bn2 = hscal[P_BNORM2];
         const double rn2 = hscal[P_RNORM2];
         if (bn2 == 0.) {
@@ -863,7 +890,7 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
         double last = r0;
         bool conv = false;
         R.iters += pcg_loop(F, itol * itol * r0, max_inner, false, red, hscal + 8, hscal_dev + 8, hpart, hpart_dev, ev, st, r0, &last,
-                            &conv);
+                            &conv, n_rz_pre);
         corr_pending = true;  // x += e, in the next sweep's residual kernel
     }
     (void)D;

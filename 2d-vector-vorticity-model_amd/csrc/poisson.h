@@ -114,6 +114,7 @@ class Poisson {
     NetScratch gscratch;       // this solver's network activations
     void guess_features(float *dst, hipStream_t st);
     hipEvent_t ev = nullptr;
+    hipEvent_t ev_res = nullptr;  // after the first residual: the host waits on it, not on the whole stream
     // Work folded into the next solve's first residual kernel (one launch
     // instead of two each): the ||b||^2 partials the right-hand-side kernel
     // left in row 1 of red (their block count, 0 for none), the guess
