@@ -75,6 +75,10 @@ struct vvm_dev {
     bool rad_on = false;
     int radcool = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mp[2] = {};
+    // the learned guess (right-hand side, features, network) on the second
+    // stream beside the scalar advection: forked after pzeta, joined before
+    // the solve
+    hipEvent_t ev_fork0 = nullptr, ev_guess = nullptr;
 
     // per-kernel event timer (vvm_dev_kernel_timer)
     static constexpr int kKtMax = 256;
@@ -248,15 +252,23 @@ int ensure_poisson(vvm_dev *h, const double *rhou, const double *rhow) {
     return VVM_OK;
 }
 
-int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss) {
+// cal_w's right-hand side and the initial guess, on stream st.
+void prepare_guess(vvm_dev *h, const StepPtrs &S, hipStream_t st) {
     const vvm_dev_config &c = h->cfg;
-    h->pois.rhs_from_zetap(h->G, S.fut[F_ZETA], h->P.rhow, h->st);
-    h->pois.collect_features(h->st);  // no-op unless recording samples
+    h->pois.rhs_from_zetap(h->G, S.fut[F_ZETA], h->P.rhow, st);
+    h->pois.collect_features(st);  // no-op unless recording samples
     h->pois.net = h->guess_net;
-    const bool learned = (c.guess_mode == VVM_GUESS_LEARNED) && h->pois.learned_guess(h->st);
+    const bool learned = (c.guess_mode == VVM_GUESS_LEARNED) && h->pois.learned_guess(st);
     // without a network (or before two solutions exist) the learned mode
     // starts from the extrapolation
-    if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP : c.guess_mode, h->st);
+    if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP : c.guess_mode, st);
+}
+
+// prepared: prepare_guess has already run (on the second stream, which the
+// main stream has waited for)
+int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss, bool prepared = false) {
+    const vvm_dev_config &c = h->cfg;
+    if (!prepared) prepare_guess(h, S, h->st);
     SolveResult R;
     const int maxit = c.max_iter > 0 ? c.max_iter : 200;
     if (c.solver_mode == VVM_SOLVER_BICGSTAB) {
@@ -373,7 +385,9 @@ int vvm_dev_create(const vvm_dev_config *cfg, int device, vvm_dev_t **out) {
     }
     for (auto &e : h->ev) ok = ok && hipEventCreate(&e) == hipSuccess;
     for (auto &e : h->ev_mp) ok = ok && hipEventCreate(&e) == hipSuccess;
-    ok = ok && hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess &&
+    ok = ok && hipEventCreateWithFlags(&h->ev_fork0, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&h->ev_guess, hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) == hipSuccess &&
          hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) == hipSuccess;
     // the second stream is created on the first step that uses it: HIP hands
     // streams to its few hardware queues in creation order, and a stream
@@ -400,7 +414,7 @@ void vvm_dev_destroy(vvm_dev_t *h) {
         if (e) hipEventDestroy(e);
     for (auto &e : h->kt_ev)
         if (e) hipEventDestroy(e);
-    for (hipEvent_t e : {h->ev_fork, h->ev_join, h->ev_mp[0], h->ev_mp[1]})
+    for (hipEvent_t e : {h->ev_fork0, h->ev_guess, h->ev_fork, h->ev_join, h->ev_mp[0], h->ev_mp[1]})
         if (e) hipEventDestroy(e);
     if (h->side) hipStreamDestroy(h->side);
     if (h->st) hipStreamDestroy(h->st);
@@ -665,11 +679,24 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         }
         StepPtrs S = ptrs(h);
         const bool last = (n == nsteps - 1) && timing;
+        if (h->overlap_c && !h->mp_fn && !h->side)
+            HIPCHK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+        const bool overlap = h->overlap_c && !h->mp_fn;
+        // the learned guess's network is the longest serial chain before the
+        // solve; it needs only pzeta's zetap and the past solutions
+        const bool early_guess = overlap && c.guess_mode == VVM_GUESS_LEARNED && h->guess_net &&
+                                 h->pois.sample_cap == 0;
         if (last) hipEventRecord(h->ev[0], h->st);
         // A: advection, buoyancy, forcing, halos (src/Iteration.cpp:263-285)
         {
             KTimer t(h, VVM_KT_PZETA);
             launch_pzeta(G, S, h->step, h->st);
+        }
+        if (early_guess) {
+            hipEventRecord(h->ev_fork0, h->st);
+            hipStreamWaitEvent(h->side, h->ev_fork0, 0);
+            prepare_guess(h, S, h->side);
+            hipEventRecord(h->ev_guess, h->side);
         }
         {
             KTimer t(h, VVM_KT_THERMO4);
@@ -687,9 +714,6 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         // C: Kessler + NegativeValueProcess x3 + halos (:303-311, 404), on
         // the side stream while B runs; the main stream waits for it before D
         // (with host microphysics, C runs after B on the host instead)
-        if (h->overlap_c && !h->mp_fn && !h->side)
-            HIPCHK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
-        const bool overlap = h->overlap_c && !h->mp_fn;
         hipStream_t cs = overlap ? h->side : h->st;
         if (overlap) {
             hipEventRecord(h->ev_fork, h->st);
@@ -714,7 +738,8 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         int rc;
         {
             KTimer t(h, VVM_KT_SOLVE);
-            rc = solve_w(h, S, &ss);
+            if (early_guess) hipStreamWaitEvent(h->st, h->ev_guess, 0);
+            rc = solve_w(h, S, &ss, early_guess);
         }
         if (rc == VVM_WARN_NOT_CONVERGED) {
             warn = rc;
