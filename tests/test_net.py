@@ -239,7 +239,19 @@ def test_learned_guess_gain():
         e1 = net.eval(held)
         held.close()
         _, s_ex = _run(7, 50, V.VVM_GUESS_EXTRAP)
-        _, s_le = _run(7, 50, V.VVM_GUESS_LEARNED, net=net)
+        d_le, s_le = _run(7, 50, V.VVM_GUESS_LEARNED, net=net)
+        # while samples are recorded the guess runs in stream order; without,
+        # on the second stream beside the scalar advection: the same step
+        d_in, s_in = _run(7, 50, V.VVM_GUESS_LEARNED, net=net, collect=50)
+        fields = []
+        for d in (d_le, d_in):
+            st = V.init_state(V.grid_case(512, 64, seed=7))
+            d.download(st)
+            fields.append(st)
+            d.close()
+        for f in ("zeta", "th", "qv", "qc", "qr", "u", "w"):
+            np.testing.assert_array_equal(fields[0][f], fields[1][f], err_msg=f)
+        assert s_in.solve_iters_total == s_le.solve_iters_total
         print(f"loss {ts.loss_first:.4g} -> {ts.loss_last:.4g} in {ts.seconds:.2f} s; held-out error "
               f"relative to extrapolation {e0:.4f} -> {e1:.4f}; iterations over 50 steps: extrapolation "
               f"{s_ex.solve_iters_total}, learned {s_le.solve_iters_total}", flush=True)
