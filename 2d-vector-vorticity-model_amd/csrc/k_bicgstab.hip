@@ -176,6 +176,7 @@ bool bicgstab_supported(int N) { return N % 2 == 0; }
 
 SolveResult Poisson::solve_bicgstab(double tol, bool absolute, int max_iter, hipStream_t st) {
     SolveResult R;
+    guess_from_net = false;  // BiCGSTAB starts from whatever guess x holds
     const size_t n = size_t(M + 2) * N;
     if (!bcg) {
         // r^, p, v, s, t, p^, s^ (zeroed, so their Dirichlet rows are zero)

@@ -24,14 +24,18 @@ inline dim3 ggrid(int N, int M) { return dim3((N + GX - 1) / GX, (M + GY - 1) / 
 
 __device__ inline double xe_at(const double *x1, const double *x2, size_t a) { return 2. * x1[a] - x2[a]; }
 
+// Partials: the three features' sums of squares, and ||wD r||^2 of the
+// extrapolated guess (the solver's own norm, wD = rhow: the reference
+// system's residual), against which the solve judges the network's guess.
 __global__ void __launch_bounds__(GX *GY) k_guess_features(int N, int M, const double *__restrict__ f,
                                                            const double *__restrict__ x1,
                                                            const double *__restrict__ x2,
                                                            const double *__restrict__ cx,
-                                                           const double *__restrict__ cu, float *__restrict__ feat,
+                                                           const double *__restrict__ cu,
+                                                           const double *__restrict__ wD, float *__restrict__ feat,
                                                            RedSlot rs) {
     const int i = blockIdx.x * GX + threadIdx.x, j = 1 + blockIdx.y * GY + threadIdx.y;
-    double part[3] = {0., 0., 0.};
+    double part[4] = {0., 0., 0., 0.};
     if (i < N && j <= M) {
         const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
         const size_t r = size_t(j) * N;
@@ -46,8 +50,10 @@ __global__ void __launch_bounds__(GX *GY) k_guess_features(int N, int M, const d
         part[0] = res * res;
         part[1] = d * d;
         part[2] = c * c;
+        const double wr = wD[j] * res;
+        part[3] = wr * wr;
     }
-    block_partials<3>(part, rs);
+    block_partials<4>(part, rs);
 }
 
 // sums of squares -> scales (0 for an all-zero feature)
@@ -83,12 +89,12 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
     }
     if (!dst) dst = gfeat;  // in place: each element reads and writes its own slot
     const dim3 g = ggrid(N, M);
-    // rows 2-4 of the reduction slots: row 1 may hold the pending ||b||^2
+    // rows 2-5 of the reduction slots: row 1 may hold the pending ||b||^2
     // partials of the right-hand side
     const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
     hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
-                       pcg64.mg.lev[0].cu, gfeat, rf);
-    finalize<3>(rf, nblocks(g), gsc, RED_STORE, 0, st);
+                       pcg64.mg.lev[0].cu, pcg64.wD, gfeat, rf);
+    finalize<4>(rf, nblocks(g), gsc, RED_STORE, 0, st);
     hipLaunchKernelGGL(k_guess_normalize, dim3(unsigned((3 * plane + 255) / 256)), dim3(256), 0, st, plane,
                        double(plane), gsc, gfeat, dst);
 }
@@ -104,17 +110,20 @@ bool Poisson::learned_guess(hipStream_t st) {
     // scales them, its last writes the guess into x (rows 1..M; the
     // Dirichlet rows of x stay zero)
     const dim3 g = ggrid(N, M);
-    // rows 2-4 of the reduction slots: row 1 may hold the pending ||b||^2
+    // rows 2-5 of the reduction slots: row 1 may hold the pending ||b||^2
     // partials of the right-hand side
     const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
     hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
-                       pcg64.mg.lev[0].cu, gfeat, rf);
-    finalize<3>(rf, nblocks(g), gsc, RED_STORE, 0, st);
+                       pcg64.mg.lev[0].cu, pcg64.wD, gfeat, rf);
+    // the extrapolated guess's residual norm goes to the host as well: the
+    // solve keeps the network's guess only if it starts lower
+    finalize<4>(rf, nblocks(g), gsc, RED_STORE, 0, st, -1, hscal_dev + H_GUESS);
     const double npts = double(plane);
     const NetGeo geo{1, M, N, 1};
     const NetFeatIn fin{gfeat, gsc, npts};
     const NetApply ap{xprev, xprev2, gsc, pcg64.x, N, npts};
-    return net->infer_guess(geo, fin, ap, gscratch, st) == VVM_OK;
+    guess_from_net = net->infer_guess(geo, fin, ap, gscratch, st) == VVM_OK;
+    return guess_from_net;
 }
 
 int Poisson::collect_reserve(int cap) {

@@ -23,6 +23,7 @@
 // channel-major in LDS; partial sums per workgroup are reduced by
 // k_wgrad_reduce.
 #include <algorithm>
+#include <mutex>
 #include <cstdint>
 
 #include "net.h"
@@ -545,22 +546,35 @@ static bool g_layer_pairs = false;
 void net_set_layer_pairs(bool on) { g_layer_pairs = on; }
 bool net_layer_pairs() { return g_layer_pairs; }
 
+// Workgroups of one k_conv instantiation that fit on the current device at
+// once, queried once per device and instantiation.  Several host threads
+// (ensemble members sharing one network) may launch at the same time, so the
+// table is filled under a lock.
+template <int MODE, int IN, int FTH>
+static long conv_resident() {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static long cached[kMaxDevices] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) dev = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    if (cached[dev] == 0) {
+        int ncu = 0, per_cu = 0;
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_conv<MODE, IN, FTH>),
+                                                     256, 0);
+        cached[dev] = long(std::max(per_cu, 1)) * std::max(ncu, 1);
+    }
+    return cached[dev];
+}
+
 template <int MODE, int IN, int FTH>
 static void conv_launch(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, __bf16 *A,
                         __bf16 *Z, float *F, const __bf16 *zb, const NetFeatIn &fin, const NetApply &ap,
                         hipStream_t st) {
     // resident workgroups: as many as fit on the device at once, at most one a tile
-    static int per_cu = 0, ncu = 0;
-    if (!per_cu) {
-        int dev = 0;
-        hipGetDevice(&dev);
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k_conv<MODE, IN, FTH>),
-                                                     256, 0);
-        if (per_cu < 1) per_cu = 1;
-    }
     const long ntile = long((g.w + FTW - 1) / FTW) * ((g.h + FTH - 1) / FTH) * g.nb;
-    const dim3 grid(unsigned(std::min<long>(ntile, long(per_cu) * ncu)));
+    const dim3 grid(unsigned(std::min<long>(ntile, conv_resident<MODE, IN, FTH>())));
     hipLaunchKernelGGL((k_conv<MODE, IN, FTH>), grid, dim3(256), 0, st, g, in,
                        reinterpret_cast<const bf16x8 *>(wpk), bias, A, Z, F, zb, fin, ap);
 }

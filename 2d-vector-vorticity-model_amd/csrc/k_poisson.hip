@@ -570,6 +570,7 @@ int pcg_loop(PCG<T> &P, double target, int max_iter, bool stall_stop, const RedS
             rn = *static_cast<volatile double *>(hs);
         }
         *rn2 = rn;
+        if (!std::isfinite(rn)) break;  // breakdown (NaN or Inf): stop, not converged
         if (!(rn > target)) {
             *conv = true;
             break;
@@ -793,6 +794,7 @@ void Poisson::back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStr
 
 void Poisson::set_guess(int mode, hipStream_t st) {
     (void)st;
+    guess_from_net = false;
     // formed by the solve's first residual kernel (first_residual)
     x_pending = XF_NONE;
     if (mode == VVM_GUESS_ZERO)
@@ -811,16 +813,41 @@ void Poisson::remember_solution(hipStream_t st) {
     nsolves++;
 }
 
+template <typename T>
+void Poisson::start_solve(double *r, bool fp32_rhs, PCG<T> *pre_vcycle, int *n_rz_pre, double *bn2, double *rn2,
+                          hipStream_t st) {
+    const bool judge = guess_from_net;
+    guess_from_net = false;
+    for (int attempt = 0;; attempt++) {
+        first_residual(r, fp32_rhs, st);
+        hipEventRecord(ev_res, st);
+        if (pre_vcycle) *n_rz_pre = pcg_first_vcycle(*pre_vcycle, red, hpart_dev, st);
+        hipEventSynchronize(ev_res);
+        *bn2 = hscal[P_BNORM2];
+        *rn2 = hscal[P_RNORM2];
+        if (!judge || attempt > 0) return;
+        const double rn2_xe = hscal[H_GUESS + 3];
+        if (std::isfinite(*rn2) && *rn2 <= rn2_xe) return;
+        // the network's guess starts worse than the extrapolation it corrects
+        // (or is not finite): start from the extrapolation instead; the first
+        // V-cycle queued on the rejected residual is simply overwritten
+        guess_fallbacks++;
+        x_pending = XF_EXTRAP;
+    }
+}
+
 SolveResult Poisson::solve_fp64(double tol, int max_iter, hipStream_t st) {
     SolveResult R;
     PCG<double> &P = pcg64;
-    first_residual(P.r, false, st);
     // the first V-cycle goes in before the host reads the residual (a guess
     // that already meets the stop rule wastes it, which is rare)
-    hipEventRecord(ev_res, st);
-    const int n_rz_pre = pcg_first_vcycle(P, red, hpart_dev, st);
-    hipEventSynchronize(ev_res);
-    const double bn2 = hscal[P_BNORM2], rn2 = hscal[P_RNORM2];
+    int n_rz_pre = 0;
+    double bn2 = 0., rn2 = 0.;
+    start_solve(P.r, false, &P, &n_rz_pre, &bn2, &rn2, st);
+    if (!std::isfinite(bn2) || !std::isfinite(rn2)) {  // a non-finite right-hand side or guess
+        R.rel0 = R.rel = std::sqrt(rn2 / bn2);
+        return R;
+    }
     if (bn2 == 0.) {
         hipMemsetAsync(P.x, 0, size_t(M + 2) * N * sizeof(double), st);
         R.converged = true;
@@ -854,16 +881,18 @@ SolveResult Poisson::solve_mixed(double tol, int max_outer, int max_inner, doubl
     PCG<float> &F = pcg32;
     double bn2 = -1.;
     for (int outer = 0; outer <= max_outer; outer++) {
-        // r64, and its FP32 copy for the correction solve (unused if converged)
-        first_residual(r64, true, st);
+        // r64, and its FP32 copy for the correction solve (unused if converged);
         // on the first sweep the inner solve's first V-cycle goes in before
         // the host reads the residual (the guess almost never meets the stop
         // rule); on later sweeps the residual is usually the final check
-        hipEventRecord(ev_res, st);
-        const int n_rz_pre = (outer == 0) ? pcg_first_vcycle(F, red, hpart_dev, st) : -1;
-        hipEventSynchronize(ev_res);
-        bn2 = hscal[P_BNORM2];
-        const double rn2 = hscal[P_RNORM2];
+        int n_rz_pre = -1;
+        double rn2 = 0.;
+        start_solve(r64, true, outer == 0 ? &F : static_cast<PCG<float> *>(nullptr), &n_rz_pre, &bn2, &rn2, st);
+        if (!std::isfinite(bn2) || !std::isfinite(rn2)) {  // a non-finite right-hand side, guess or correction
+            R.rel = std::sqrt(rn2 / bn2);
+            if (outer == 0) R.rel0 = R.rel;
+            return R;
+        }
         if (bn2 == 0.) {
             hipMemsetAsync(D.x, 0, size_t(M + 2) * N * sizeof(double), st);
             R.converged = true;

@@ -86,6 +86,13 @@ class Poisson {
     // x_{t-2}; returns false (nothing written) otherwise.
     GuessNet *net = nullptr;
     bool learned_guess(hipStream_t st);
+    // The network's guess was written for the next solve.  That solve keeps
+    // it only if its initial residual is finite and below the extrapolated
+    // guess's (which the feature pass measured); otherwise it restarts from
+    // the extrapolation, so a poor or broken network never costs iterations
+    // or accuracy.  guess_fallbacks counts the restarts.
+    bool guess_from_net = false;
+    int guess_fallbacks = 0;
     // Sample recording for training: features before the solve, target
     // (x - xe) / s_d after it.  cap 0 frees the buffer.
     int collect_reserve(int cap);
@@ -105,7 +112,11 @@ class Poisson {
     double *xprev = nullptr;   // previous solution (for extrapolation)
     double *xprev2 = nullptr;
     double *r64 = nullptr;     // FP64 residual of the refinement loop
-    double *hscal = nullptr;   // pinned host copy of scalars
+    // pinned host copy of scalars: [P_RNORM2], [P_BNORM2] of the first
+    // residual, [8..] the CG loop's, [H_GUESS..H_GUESS+3] the feature pass's
+    // sums (H_GUESS + 3: ||wD r||^2 of the extrapolated guess)
+    static constexpr int H_GUESS = 12;
+    double *hscal = nullptr;
     double *hscal_dev = nullptr;  // the same memory as the device sees it
     double *hpart = nullptr, *hpart_dev = nullptr;  // per-workgroup norms (mapped host memory)
     RedSlot red;
@@ -132,6 +143,13 @@ class Poisson {
     // written), ||w r||^2 (and ||b||^2 if pending) into sc and hscal; with
     // fp32_rhs also the FP32 copy of r and a zero FP32 x.
     void first_residual(double *r, bool fp32_rhs, hipStream_t st);
+    // The first residual of a solve, with its first V-cycle queued before the
+    // host reads the norms (pre_vcycle: that V-cycle's partial count into
+    // *n_rz_pre), and the learned guess judged against the extrapolation.
+    // Returns ||b||^2 and ||w r||^2 as the host read them.
+    template <typename T>
+    void start_solve(double *r, bool fp32_rhs, PCG<T> *pre_vcycle, int *n_rz_pre, double *bn2, double *rn2,
+                     hipStream_t st);
 };
 
 // cal_u on the device (src/PoissonSolver.cpp:580-653): top-row potential

@@ -667,6 +667,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
     const Grid &G = h->G;
     const vvm_dev_config &c = h->cfg;
     vvm_step_stats acc{};
+    const int fallbacks0 = h->pois.guess_fallbacks;
     int warn = 0;
     const bool timing = (stats != nullptr);
     auto t0 = std::chrono::steady_clock::now();
@@ -746,6 +747,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             acc.not_converged++;
         }
         acc.solve_iters_total += ss.iters;
+        acc.guess_fallbacks = h->pois.guess_fallbacks - fallbacks0;
         acc.solve_iters_last = ss.iters;
         acc.refine_last = ss.refine;
         acc.rel_residual_last = ss.rel_residual;

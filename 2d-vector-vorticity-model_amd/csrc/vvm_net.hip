@@ -155,7 +155,7 @@ int GuessNet::infer_guess(const NetGeo &g, const NetFeatIn &fin, const NetApply 
             cur ^= 1;
         }
     }
-    return VVM_OK;
+    return hipGetLastError() == hipSuccess ? VVM_OK : VVM_ERR_HIP;
 }
 
 int GuessNet::forward(const NetGeo &g, const float *feat, float *out, bool keep, hipStream_t s) {

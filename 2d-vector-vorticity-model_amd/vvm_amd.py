@@ -23,6 +23,7 @@ LIB_PATH = os.path.join(PKG_DIR, "libvvm_amd.so")
 VVM_SOLVER_FP64, VVM_SOLVER_MIXED, VVM_SOLVER_BICGSTAB = 0, 1, 2
 VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED = 0, 1, 2, 3
 VVM_STOP_PARITY, VVM_STOP_REL, VVM_STOP_ABS = 0, 1, 2
+VVM_OK, VVM_WARN_NOT_CONVERGED = 0, 1
 KT = {"none": 0, "pzeta": 1, "thermo4": 2, "scale": 3, "kessler": 4, "turb": 5, "diffusion": 6, "mean": 7,
       "calu": 8, "solve": 9}
 
@@ -72,7 +73,7 @@ class VvmStepStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("solve_iters_total", C.c_int32), ("solve_iters_last", C.c_int32),
                 ("refine_last", C.c_int32), ("not_converged", C.c_int32), ("rel_residual_last", C.c_double),
                 ("t_advection", C.c_double), ("t_poisson", C.c_double), ("t_microphysics", C.c_double),
-                ("t_diffusion", C.c_double), ("t_all", C.c_double)]
+                ("t_diffusion", C.c_double), ("t_all", C.c_double), ("guess_fallbacks", C.c_int32)]
 
 
 class VvmSolveStats(C.Structure):
@@ -486,8 +487,10 @@ class Device:
     def poisson(self, b: np.ndarray, x: np.ndarray) -> VvmSolveStats:
         s = VvmSolveStats()
         assert b.dtype == np.float64 and x.dtype == np.float64 and b.flags.c_contiguous and x.flags.c_contiguous
-        self._check(self.lib.vvm_dev_poisson(self.h, b.ctypes.data_as(PTR), x.ctypes.data_as(PTR), C.byref(s)),
-                    "poisson")
+        # VVM_WARN_NOT_CONVERGED (1) is kept in last_rc, as the reference
+        # warns and goes on (src/PoissonSolver.cpp:463-467)
+        self.last_rc = self._check(self.lib.vvm_dev_poisson(self.h, b.ctypes.data_as(PTR), x.ctypes.data_as(PTR),
+                                                            C.byref(s)), "poisson")
         return s
 
     def poisson_device(self, d_b: int, d_x: int) -> VvmSolveStats:

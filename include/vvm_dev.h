@@ -119,6 +119,9 @@ typedef struct vvm_step_stats {
     int32_t not_converged;     /* solves that hit max_iter                   */
     double rel_residual_last;  /* ||b-Ax||/||b|| of the last solve          */
     double t_advection, t_poisson, t_microphysics, t_diffusion, t_all; /* s */
+    int32_t guess_fallbacks;   /* solves whose learned guess started worse
+                                  than the extrapolation (or not finite) and
+                                  were restarted from the extrapolation    */
 } vvm_step_stats;
 
 typedef struct vvm_solve_stats {

@@ -45,6 +45,12 @@ def step(cfg, st, nsteps: int = 1, gsolve: int = GSOLVE_EXACT) -> None:
     _call("vo_step", cfg, st, C.c_int(nsteps), C.c_int(gsolve))
 
 
+def random_perturbation(cfg, st, t: int, lo: float = -0.001, hi: float = 0.001, sd: float = 1.0) -> None:
+    """Init::RandomPerturbation into st['init_th_forcing'] (the oracle's own
+    restatement, oracle/vvm_oracle_rng.cpp)."""
+    _call("vo_random_perturbation", cfg, st, C.c_int(t), C.c_double(lo), C.c_double(hi), C.c_double(sd))
+
+
 OPS = {
     "pzeta_pt": "vo_pzeta_pt", "pth_pt": "vo_pth_pt", "pqv_pt": "vo_pqv_pt", "pqc_pt": "vo_pqc_pt",
     "pqr_pt": "vo_pqr_pt", "forcing_bc": "vo_forcing_bc", "bc_all": "vo_bc_all",

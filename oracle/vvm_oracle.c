@@ -868,6 +868,10 @@ int vo_step(const vvm_dev_config *c, vvm_host_view *v, int nsteps, int gsolve) {
         vo_pqc_pt(c, v);
         vo_pqr_pt(c, v);
         v->status_for_adding_forcing = (v->step * c->dt <= c->addforcingtime) ? 1 : 0;
+        /* a new random theta kick every step it is applied
+         * (src/Iteration.cpp:276-282) */
+        if (v->status_for_adding_forcing && vo_random_perturbation(c, v, v->step + 0, -0.001, 0.001, 1.) != 0)
+            return -1;
         vo_forcing_bc(c, v);
         vo_pubartop_pt(c, v);
         if (vo_cal_w(c, v) != 0) return -1;

@@ -37,9 +37,14 @@ int vo_nudge(const vvm_dev_config *c, vvm_host_view *v);
 int vo_update_mean(const vvm_dev_config *c, vvm_host_view *v);
 int vo_next_time_step(const vvm_dev_config *c, vvm_host_view *v);
 
+/* Init::RandomPerturbation (src/Init.cpp:488-515): init_th_forcing from
+ * std::mt19937(t) and std::normal_distribution (vvm_oracle_rng.cpp). */
+int vo_random_perturbation(const vvm_dev_config *c, vvm_host_view *v, int t, double min_range,
+                           double max_range, double stddev);
+
 /* nsteps iterations of the TimeMarching loop body starting from v->step.
- * The random theta kick (addforcingtime > 0) uses v->init_th_forcing as
- * supplied by the caller. */
+ * While step*dt <= addforcingtime each step draws a new random theta kick
+ * into v->init_th_forcing (src/Iteration.cpp:276-282) before AddForcing. */
 int vo_step(const vvm_dev_config *c, vvm_host_view *v, int nsteps, int gsolve);
 
 /* Exact solve of cal_w's system A x = b (b, x in the reference's row order

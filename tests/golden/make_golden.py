@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "2d-vector-vorticity-model_amd"), os.path.join(ROOT, "tests")]
 
 import vvm_amd as V  # noqa: E402
-from helpers import perturbed_state  # noqa: E402
+from helpers import branch_cases, input_checksums, perturbed_state  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 REF_OPS = ["pzeta_pt", "pth_pt", "pqv_pt", "pqc_pt", "pqr_pt", "forcing_bc", "bc_all", "pubarTop_pt", "cal_w",
@@ -165,12 +165,43 @@ def init_fixture(tmp):
     np.savez_compressed(os.path.join(HERE, "init_ref.npz"), **data)
 
 
+BRANCH_FIELDS = ["zeta", "th", "qv", "qc", "qr", "u", "w", "RKM", "RKH", "init_th_forcing"]
+
+
+def reference_init(binary: str, cfg, tmp) -> dict:
+    """Init1d + Init2d as the compiled reference runs them on cfg's grid."""
+    st = V.HostState(cfg.nx, cfg.nz)
+    pin, pout = os.path.join(tmp, "init_in.state"), os.path.join(tmp, "init_out.state")
+    st.save(pin, cfg)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    subprocess.run([os.path.join(ROOT, "oracle", "_ref", binary), "init", "-", pin, pout], check=True, env=env,
+                   timeout=120)
+    return V.HostState.read_records(pout)
+
+
+def branch_fixture(tmp):
+    data = {}
+    for name, (cfg, st, n) in branch_cases().items():
+        if name == "refdefault":
+            # the state stepped below is the reference's own initial state
+            r = reference_init("vvm_ref_load", cfg, tmp)
+            for f, a in r.items():
+                if f in st.a and not np.array_equal(st.a[f], a.reshape(st.a[f].shape)):
+                    sys.exit(f"refdefault: the host init differs from the reference's in {f}")
+        data[f"{name}/in_checksum"] = input_checksums(st)
+        r = run_ref("step", n, st, cfg, tmp)
+        for f in BRANCH_FIELDS:
+            data[f"{name}/{n}/{f}"] = r[f]
+        data[f"{name}/steps"] = np.array([n])
+    np.savez_compressed(os.path.join(HERE, "branches.npz"), **data)
+
+
 def main():
     if not O.ref_available():
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "ref", "build_ref.sh")], check=True)
     if not O.ref_available():
         sys.exit("reference binary unavailable")
-    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long", "init"}
+    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long", "init", "branches"}
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as tmp:
         if "ops" in which:
             ops_fixture(tmp)
@@ -182,6 +213,8 @@ def main():
             long_fixture(tmp)
         if "init" in which:
             init_fixture(tmp)
+        if "branches" in which:
+            branch_fixture(tmp)
 
 
 if __name__ == "__main__":

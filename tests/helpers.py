@@ -1,6 +1,8 @@
 """Shared test helpers: seeded perturbed model states and field comparisons."""
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 import vvm_amd as V
@@ -87,3 +89,40 @@ def gpu_run_op(dev: V.Device, st: V.HostState, op: str) -> V.HostState:
     dev.op(op)
     dev.download(out)
     return out
+
+
+SOUNDING = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "init_sounding.txt")
+
+
+def branch_cases():
+    """The step's branches that the benchmark cases leave off, each on a
+    grid deep enough for the nudged layer (z > 12 km) where that matters:
+      kick_fine   -- the per-step random theta kick (addforcingtime = 10 s,
+                     the reference's own value, src/main.cpp:70) on a 50 m
+                     grid with four kicked levels below 200 m;
+      kick_c1     -- the kick on a c1-style 200 m grid (steps 1-5 kicked);
+      nudge_qv    -- Nudge_qv with moisture_nudge_time = 3600 s
+                     (src/NumericalProcess.cpp:87-96);
+      case2       -- CASE 2, which skips Nudge_zeta (src/Iteration.cpp:416);
+      refdefault  -- the reference's default run: Init1d from the sounding
+                     file input/init.txt, Init2d's TROPICALFORCING branch,
+                     dt = 3 s, dx = dz = 200 m, addforcingtime = 10 s.
+    Returns name -> (cfg, initial HostState, steps)."""
+    out = {}
+    c = V.grid_case(24, 16, dx=50.0, dt=1.0, addforcingtime=10.0)
+    out["kick_fine"] = (c.cfg, V.init_state(c), 12)
+    c = V.grid_case(32, 64, dx=200.0, dt=2.0, addforcingtime=10.0)
+    out["kick_c1"] = (c.cfg, V.init_state(c), 8)
+    c = V.grid_case(32, 64, dx=200.0, dt=2.0, moisture_nudge_time=3600.0)
+    out["nudge_qv"] = (c.cfg, perturbed_state(c, seed=31, step=0), 8)
+    c = V.grid_case(32, 64, dx=200.0, dt=2.0, CASE=2)
+    out["case2"] = (c.cfg, perturbed_state(c, seed=32, step=0), 8)
+    cfg = V.make_config(32, 100, 200.0, 200.0, 3.0, CASE=1, addforcingtime=10.0, tropical_forcing=1)
+    out["refdefault"] = (cfg, V.init_reference(cfg, SOUNDING), 10)
+    return out
+
+
+def input_checksums(st: V.HostState) -> np.ndarray:
+    """Per-array sums of |value| of a state, in sorted name order: the
+    tests regenerate each initial state and compare these first."""
+    return np.array([float(np.abs(st.a[n]).sum()) for n in sorted(st.a)])

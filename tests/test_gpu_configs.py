@@ -1,0 +1,214 @@
+"""The benchmark workloads at their full sizes on the GPU, and the step's
+branches the benchmark cases leave off, against the CPU oracle.
+
+  - branches: the per-step random theta kick at the reference's own
+    addforcingtime = 10 s (src/main.cpp:70, src/Iteration.cpp:276-282),
+    Nudge_qv (src/NumericalProcess.cpp:87-96), CASE 2's Nudge_zeta skip
+    (src/Iteration.cpp:416) and the reference's default run from
+    input/init.txt; the oracle is pinned to the compiled reference on the
+    same cases (tests/golden/branches.npz).  Bar: 1e-10 relative L2 per field.
+  - config 2 (2048 x 256, dx = dz = 100 m, dt = 1 s): two FP64 steps at the
+    parity stop rule against the oracle (1e-10 per field); 20 steps of each
+    benchmark path (FP64 from the zero guess, mixed precision from the
+    extrapolated guess, mixed precision from the learned guess), every solve
+    at ||b - Ax|| / ||b|| <= 1e-10 and every field within 1e-6 relative L2
+    of the FP64 oracle.
+  - config 4: the bench's 64-member draw of uneven grids (nx in {256, 512,
+    1024, 2048}, nz in {64, 128, 256}), 300 steps mixed precision with the
+    learned guess and tuned launch shapes against the same ensemble in FP64
+    (1e-6 per member and field), and three members against the oracle.
+"""
+import numpy as np
+import pytest
+
+import vvm_amd as V
+from helpers import COMPARE_STEP, assert_fields_close, branch_cases, rel_err
+
+pytestmark = pytest.mark.gpu
+
+PROG = ["zeta", "th", "qv", "qc", "qr", "u", "w"]
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def run_device(cfg, st0, n, net=None, per_step=None):
+    """n steps of st0 on a fresh handle; per_step(stats) after every step
+    when given (one vvm_dev_step call per step), else one batch."""
+    with V.Device(cfg) as dev:
+        dev.upload(st0)
+        if net is not None:
+            dev.set_guess_net(net)
+        total = []
+        if per_step is None:
+            total.append(dev.step(n))
+        else:
+            for _ in range(n):
+                s = dev.step(1)
+                per_step(s)
+                total.append(s)
+        got = st0.copy()
+        dev.download(got)
+    return got, total
+
+
+# ----------------------------------------------------------- branches
+
+@pytest.fixture(scope="module")
+def branches():
+    return branch_cases()
+
+
+@pytest.mark.parametrize("name", ["kick_fine", "kick_c1", "nudge_qv", "case2", "refdefault"])
+def test_branch_steps_device_vs_oracle(oracle, branches, name):
+    cfg, st0, n = branches[name]
+    log(f"branch {name}: {cfg.nx - 2}x{cfg.nz - 2}, {n} steps")
+    want = st0.copy()
+    oracle.step(cfg, want, n)
+    got, _ = run_device(cfg, st0, n)
+    assert got.step == want.step == n
+    assert_fields_close(got, want, COMPARE_STEP + ["init_th_forcing"], 1e-10, where=f"{name} {n} steps")
+
+
+# ------------------------------------------------------------ config 2
+
+@pytest.fixture(scope="module")
+def c2_oracle(oracle):
+    """The FP64 oracle's config-2 trajectory after 2 and 20 steps."""
+    c = V.case("c2")
+    st = V.init_state(c)
+    o = st.copy()
+    oracle.step(c.cfg, o, 2)
+    two = o.copy()
+    oracle.step(c.cfg, o, 18)
+    return st, {2: two, 20: o}
+
+
+@pytest.fixture(scope="module")
+def quick_net():
+    """A guess network trained briefly on samples the model records from a
+    1024 x 128 run (seed 11): enough to exercise the learned path at
+    2048 x 256; the solve's stop rule, not the network, fixes the accuracy."""
+    net = V.Net()
+    net.init_params(1)
+    c = V.grid_case(1024, 128, seed=11, stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400,
+                    solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP)
+    with V.Device(c.cfg) as dev:
+        dev.upload(V.init_state(c))
+        dev.collect(60)
+        dev.step(60)
+        net.add_samples(dev)
+    ts = net.train(steps=800, batch=16, crop=(64, 64), lr=2e-3, lr_final=1e-4, seed=5)
+    log(f"quick net: {net.pool_size()} samples, loss {ts.loss_first:.3g} -> {ts.loss_last:.3g}")
+    yield net
+    net.close()
+
+
+def test_c2_two_steps_parity(c2_oracle):
+    st, traj = c2_oracle
+    c = V.case("c2")
+    log("config 2: 2 FP64 steps at the parity stop rule")
+    got, stats = run_device(c.cfg, st, 2)
+    assert stats[0].not_converged == 0
+    assert_fields_close(got, traj[2], COMPARE_STEP, 1e-10, where="c2 2 steps")
+
+
+C2_PATHS = {
+    "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
+    "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+    "mixed_learned": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_LEARNED),
+}
+
+
+@pytest.mark.parametrize("path", list(C2_PATHS))
+def test_c2_bench_paths_20_steps(c2_oracle, quick_net, path):
+    st, traj = c2_oracle
+    c = V.case("c2", stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, **C2_PATHS[path])
+    log(f"config 2, path {path}: 20 steps, every solve to 1e-10")
+    residuals = []
+
+    def check(s):
+        assert s.not_converged == 0
+        assert s.rel_residual_last <= 1e-10, s.rel_residual_last
+        residuals.append(s.rel_residual_last)
+
+    got, stats = run_device(c.cfg, st, 20, net=quick_net if path == "mixed_learned" else None, per_step=check)
+    errs = {f: rel_err(got[f], traj[20][f]) for f in PROG}
+    iters = sum(s.solve_iters_total for s in stats)
+    fb = sum(s.guess_fallbacks for s in stats)
+    log(f"  {path}: {iters} iterations over 20 steps, worst residual {max(residuals):.2e}, "
+        f"learned-guess restarts {fb}, field error vs FP64 oracle " +
+        ", ".join(f"{f} {e:.1e}" for f, e in errs.items()))
+    for f, e in errs.items():
+        assert e <= 1e-6, (path, f, e)
+
+
+# ------------------------------------------------------------ config 4
+
+def c4_draws(seed: int = 4):
+    """The bench's 64-member draw: (nx, nz, theta noise sigma, heat flux)."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.choice([256, 512, 1024, 2048])), int(rng.choice([64, 128, 256])),
+             float(rng.uniform(0.05, 0.2)), float(rng.uniform(50, 200))) for _ in range(64)]
+
+
+def c4_cases(**over):
+    return [V.grid_case(nx, nz, seed=1000 + k, sigma=sg, heat=hf, **over)
+            for k, (nx, nz, sg, hf) in enumerate(c4_draws())]
+
+
+def test_c4_ensemble_300_steps_mixed_learned_vs_fp64(quick_net):
+    steps = 300
+    fast = c4_cases(stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, solver_mode=V.VVM_SOLVER_MIXED,
+                    guess_mode=V.VVM_GUESS_LEARNED)
+    ref = c4_cases(stop_mode=V.VVM_STOP_PARITY)
+    pts = sum(nx * nz for nx, nz, _, _ in c4_draws())
+    log(f"config 4: 64 members ({pts} points), {steps} steps, mixed + learned vs FP64")
+    with V.Ensemble(fast) as ea, V.Ensemble(ref) as eb:
+        ea.set_guess_net(quick_net)
+        ea.tune(2)
+        sa = ea.step(steps)
+        sb = eb.step(steps)
+        assert sa.members == sb.members == 64
+        assert sa.not_converged == 0 and sb.not_converged == 0
+        worst = (0.0, -1, "")
+        for i, (ca, cb) in enumerate(zip(fast, ref)):
+            a, b = V.init_state(ca), V.init_state(cb)
+            ea.member(i).download(a)
+            eb.member(i).download(b)
+            for f in PROG:
+                e = rel_err(a[f], b[f])
+                if e > worst[0]:
+                    worst = (e, i, f)
+                assert e <= 1e-6, (i, ca.cfg.nx - 2, ca.cfg.nz - 2, f, e)
+    log(f"  mixed+learned {sa.grid_point_steps / sa.seconds:.4g} grid-point-steps/s, FP64 "
+        f"{sb.grid_point_steps / sb.seconds:.4g}; worst member error {worst[0]:.2e} (member {worst[1]}, {worst[2]})")
+
+
+def test_c4_members_vs_oracle(oracle):
+    """Three members of the draw (the smallest grid, a middle one and the
+    largest) against the oracle, FP64 at the parity stop rule (1e-9 per
+    field after 10 steps, the bar of the 10-step config-0 test: round-off of
+    the two solvers grows over the steps) and mixed precision (1e-6)."""
+    steps = 10
+    draws = c4_draws()
+    sizes = [nx * nz for nx, nz, _, _ in draws]
+    order = sorted(range(64), key=lambda k: sizes[k])
+    pick = [order[0], order[32], order[-1]]
+    fp64 = c4_cases(stop_mode=V.VVM_STOP_PARITY)
+    mixed = c4_cases(stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, max_iter=400, solver_mode=V.VVM_SOLVER_MIXED,
+                     guess_mode=V.VVM_GUESS_EXTRAP)
+    with V.Ensemble(fp64) as ea, V.Ensemble(mixed) as eb:
+        ea.step(steps)
+        eb.step(steps)
+        for k in pick:
+            c = fp64[k]
+            log(f"config 4 member {k}: {c.cfg.nx - 2}x{c.cfg.nz - 2}, {steps} steps against the oracle")
+            want = V.init_state(c)
+            oracle.step(c.cfg, want, steps)
+            a, b = V.init_state(c), V.init_state(mixed[k])
+            ea.member(k).download(a)
+            eb.member(k).download(b)
+            assert_fields_close(a, want, COMPARE_STEP, 1e-9, where=f"member {k} fp64")
+            assert_fields_close(b, want, PROG, 1e-6, where=f"member {k} mixed")

@@ -259,6 +259,8 @@ def test_learned_guess_gain():
     assert e1 < 0.9 * e0 and e1 < 0.9
     assert s_le.not_converged == 0 and s_le.rel_residual_last <= 1e-10
     # the solver's own inference path (raw features scaled while staged, the
-    # guess written by the last layer) must give a guess at least as good as
-    # the extrapolation: a wrong guess costs iterations in every solve
-    assert s_le.solve_iters_total <= 1.1 * s_ex.solve_iters_total, (s_le.solve_iters_total, s_ex.solve_iters_total)
+    # guess written by the last layer) must save iterations over the
+    # extrapolation it corrects; a guess that starts worse is dropped by the
+    # solve, so the gain comes from the guesses kept
+    print(f"learned guesses dropped for the extrapolation: {s_le.guess_fallbacks} of 48", flush=True)
+    assert s_le.solve_iters_total < s_ex.solve_iters_total, (s_le.solve_iters_total, s_ex.solve_iters_total)

@@ -116,3 +116,63 @@ def test_random_perturbation_matches_reference():
         v = st.view()
         assert lib.vvm_random_perturbation(C.byref(c.cfg), C.byref(v), t, -0.001, 0.001, 1.0) == 0
         assert np.array_equal(st["init_th_forcing"], d[f"rp/{t}"].reshape(st["init_th_forcing"].shape))
+
+
+def test_oracle_random_perturbation_matches_reference(oracle):
+    """The oracle's own RandomPerturbation (vvm_oracle_rng.cpp) reproduces
+    the compiled reference's stream bit for bit."""
+    d = np.load(os.path.join(GOLD, "misc.npz"))
+    c = V.grid_case(24, 16, dx=50.0, dt=1.0)
+    for t in (1, 7):
+        st = V.HostState(c.cfg.nx, c.cfg.nz)
+        oracle.random_perturbation(c.cfg, st, t)
+        assert np.array_equal(st["init_th_forcing"], d[f"rp/{t}"].reshape(st["init_th_forcing"].shape))
+
+
+BRANCHES = ["kick_fine", "kick_c1", "nudge_qv", "case2", "refdefault"]
+BRANCH_FIELDS = ["zeta", "th", "qv", "qc", "qr", "u", "w", "RKM", "RKH", "init_th_forcing"]
+
+
+@pytest.fixture(scope="module")
+def branch_gold():
+    return np.load(os.path.join(GOLD, "branches.npz"))
+
+
+@pytest.fixture(scope="module")
+def branch_states():
+    from helpers import branch_cases
+    return branch_cases()
+
+
+@pytest.mark.parametrize("name", BRANCHES)
+def test_branch_steps_vs_reference(oracle, branch_gold, branch_states, name):
+    """The step's branches the benchmark cases leave off -- the per-step
+    random kick at the reference's addforcingtime = 10 s (src/main.cpp:70,
+    src/Iteration.cpp:276-282), Nudge_qv (src/NumericalProcess.cpp:87-96),
+    CASE 2's Nudge_zeta skip (src/Iteration.cpp:416) and the reference's
+    default run from input/init.txt -- against the compiled reference."""
+    from helpers import input_checksums
+    cfg, st0, n = branch_states[name]
+    np.testing.assert_allclose(input_checksums(st0), branch_gold[f"{name}/in_checksum"], rtol=1e-15, atol=0)
+    st = st0.copy()
+    oracle.step(cfg, st, n)
+    for f in BRANCH_FIELDS:
+        want = branch_gold[f"{name}/{n}/{f}"].reshape(st[f].shape)
+        e = rel_err(st[f], want)
+        assert e <= 1e-11, f"{name}/{f}: rel {e:.3e}"
+
+
+@pytest.mark.parametrize("name,flip", [("kick_c1", "addforcingtime"), ("nudge_qv", "moisture_nudge_time"),
+                                       ("case2", "CASE")])
+def test_branch_fixture_discriminates(oracle, branch_gold, branch_states, name, flip):
+    """The fixture pins the branch: the oracle with the branch switched off
+    misses the golden by far more than the tolerance above."""
+    import copy
+    cfg, st0, n = branch_states[name]
+    cfg = copy.copy(cfg)
+    setattr(cfg, flip, {"addforcingtime": 0.0, "moisture_nudge_time": 0.0, "CASE": 1}[flip])
+    st = st0.copy()
+    oracle.step(cfg, st, n)
+    field = {"addforcingtime": "th", "moisture_nudge_time": "qv", "CASE": "zeta"}[flip]
+    want = branch_gold[f"{name}/{n}/{field}"].reshape(st[field].shape)
+    assert rel_err(st[field], want) > 1e-9
