@@ -239,12 +239,12 @@ __device__ void sweep(const Grid &G, double *__restrict__ f, const double *__res
 // blockIdx.x: 0 zeta (Mparam), 1..4 th, qv, qc, qr (Hparam).
 // nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta.
 template <int E>
-__global__ void __launch_bounds__(kMaxWaves *kWave) k_diff_sweep(Grid G, StepPtrs S, int nudge) {
+__global__ void __launch_bounds__(kMaxWaves *kWave) k_diff_sweep(Grid G, StepPtrs S, int nudge, int fbase) {
     extern __shared__ unsigned long long carry[];  // [waves][nz]
     const int waves = blockDim.x / kWave;
     for (int t = threadIdx.x; t < waves * G.nz; t += blockDim.x) carry[t] = kEmpty;
     __syncthreads();
-    const int fld = blockIdx.x;  // F_ZETA .. F_QR
+    const int fld = fbase + blockIdx.x;  // F_ZETA .. F_QR
     const bool mpar = (fld == F_ZETA);
     const double *rate = mpar ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
     sweep<E>(G, S.fut[fld], S.dcoef[mpar ? DC_AM : DC_AH], S.dcoef[DC_C0 + fld], S.dcoef[mpar ? DC_DM : DC_DH], rate,
@@ -333,7 +333,8 @@ __device__ void courier(const unsigned long long *in, unsigned long long *out, u
 }
 
 template <int E, int CW>
-__global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPtrs S, int nudge, int nstrip) {
+__global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPtrs S, int nudge, int nstrip,
+                                                                   int fbase) {
     // LDS: carry[cw][nz], cin[nz] (u64), then ghost[nz], rate[nz], thbi[nz]
     // (doubles): the per-row values a compute wave reads come from LDS, so
     // its only global loads are the prefetched coefficient rows (a global
@@ -344,7 +345,7 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
     const int nz = G.nz, n = G.nx - 2;
     unsigned long long *carry = lds, *cin = lds + size_t(cw) * nz;
     double *ghost = reinterpret_cast<double *>(lds + size_t(cw + 1) * nz), *rate_l = ghost + nz, *thbi = rate_l + nz;
-    const int s = blockIdx.x, fld = blockIdx.y;
+    const int s = blockIdx.x, fld = fbase + blockIdx.y;
     {
         const bool mz = (fld == F_ZETA);
         const double *rt = mz ? ((nudge & 2) ? S.P.rate_z : nullptr) : ((fld == F_TH && (nudge & 1)) ? S.P.rate_th : nullptr);
@@ -533,11 +534,11 @@ __global__ void __launch_bounds__((CW + 1) * kWave) k_diff_strips(Grid G, StepPt
 
 
 template <int E>
-void launch_e(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+void launch_e(const Grid &G, const StepPtrs &S, int nudge, int fbase, int nfld, hipStream_t st) {
     const int n = G.nx - 2;
     const int waves = (n + kWave * E - 1) / (kWave * E);
     const size_t lds = size_t(waves) * G.nz * sizeof(unsigned long long);
-    hipLaunchKernelGGL(k_diff_sweep<E>, dim3(NPROG), dim3(waves * kWave), lds, st, G, S, nudge);
+    hipLaunchKernelGGL(k_diff_sweep<E>, dim3(nfld), dim3(waves * kWave), lds, st, G, S, nudge, fbase);
 }
 
 }  // namespace
@@ -591,23 +592,24 @@ void diffusion_slots_release(unsigned long long *p) {
         if (r.p == p) r.busy = false;
 }
 
-void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st) {
+void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st, int only_field) {
     hipLaunchKernelGGL(k_diff_coeffs, dim3((G.nx + BX - 1) / BX, (G.nz + BY - 1) / BY), dim3(BX, BY), 0, st, G, S);
     const int n = G.nx - 2;
+    const int fbase = only_field >= 0 ? only_field : 0, nfld = only_field >= 0 ? 1 : NPROG;
     if (S.dslots) {
         const int cw = strip_waves(S);  // compute waves of one column per lane
         const int ns = strips_of(G, cw);
         const size_t lds = size_t(cw + 4) * G.nz * sizeof(unsigned long long);
-        const dim3 grid(ns, NPROG), block((cw + 1) * kWave);
-        if (cw == 2) hipLaunchKernelGGL((k_diff_strips<1, 2>), grid, block, lds, st, G, S, nudge, ns);
-        else if (cw == 8) hipLaunchKernelGGL((k_diff_strips<1, 8>), grid, block, lds, st, G, S, nudge, ns);
-        else hipLaunchKernelGGL((k_diff_strips<1, 4>), grid, block, lds, st, G, S, nudge, ns);
+        const dim3 grid(ns, nfld), block((cw + 1) * kWave);
+        if (cw == 2) hipLaunchKernelGGL((k_diff_strips<1, 2>), grid, block, lds, st, G, S, nudge, ns, fbase);
+        else if (cw == 8) hipLaunchKernelGGL((k_diff_strips<1, 8>), grid, block, lds, st, G, S, nudge, ns, fbase);
+        else hipLaunchKernelGGL((k_diff_strips<1, 4>), grid, block, lds, st, G, S, nudge, ns, fbase);
         return;
     }
-    if (n <= kMaxWaves * kWave) launch_e<1>(G, S, nudge, st);
-    else if (n <= 2 * kMaxWaves * kWave) launch_e<2>(G, S, nudge, st);
-    else if (n <= 4 * kMaxWaves * kWave) launch_e<4>(G, S, nudge, st);
-    else launch_e<8>(G, S, nudge, st);
+    if (n <= kMaxWaves * kWave) launch_e<1>(G, S, nudge, fbase, nfld, st);
+    else if (n <= 2 * kMaxWaves * kWave) launch_e<2>(G, S, nudge, fbase, nfld, st);
+    else if (n <= 4 * kMaxWaves * kWave) launch_e<4>(G, S, nudge, fbase, nfld, st);
+    else launch_e<8>(G, S, nudge, fbase, nfld, st);
 }
 
 }  // namespace vvmd

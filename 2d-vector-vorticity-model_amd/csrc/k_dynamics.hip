@@ -124,6 +124,13 @@ __device__ inline double ab2_hist(double *const (&h)[2], size_t p, int step, dou
 // differences.
 constexpr int PZX = 64, PZY = 8;
 
+// PARTS: PZ_ADV Advection_zeta, PZ_BUOY Bouyancy; both (the timestep) also
+// apply BoundaryProcess2D_westdown.  Alone each writes what the reference
+// function writes: the interior points k = 2..nz-2 (zetap = ... for the
+// advection, zetap += ... for the buoyancy) and its AB2 history.
+enum { PZ_ADV = 1, PZ_BUOY = 2, PZ_BOTH = 3 };
+
+template <int PARTS>
 __global__ void __launch_bounds__(PZX *PZY) k_pzeta(Grid G, StepPtrs S, int step) {
     __shared__ double Fx[PZY][PZX + 1];  // slot c: x-flux east of column i0 + c - 1
     __shared__ double Fz[PZY + 1][PZX];  // slot r: z-flux above row k0 + r - 1
@@ -132,32 +139,43 @@ __global__ void __launch_bounds__(PZX *PZY) k_pzeta(Grid G, StepPtrs S, int step
     const int i = i0 + tx, k = k0 + ty;
     const int nx = G.nx, nz = G.nz;
     const double *z = S.now[F_ZETA], *u = S.u, *w = S.w, *rhou = S.P.rhou, *rhow = S.P.rhow;
-    // flux_u at i-1 = 0 is the periodic image of nx-2 (BoundaryProcess2D_center
-    // on flux_u): the ghost column 0 holds it for column 1
-    if (k >= 2 && k <= nz - 2 && i <= nx - 2) Fx[ty][tx + 1] = zflux_x(G, z, u, rhou, (i == 0) ? nx - 2 : i, k);
-    if (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2) Fz[ty + 1][tx] = zflux_z(G, z, w, rhow, i, k);
-    if (ty == 0 && tx < PZY) {  // west edge: the face east of column i0 - 1
-        const int kr = k0 + tx;
-        if (i0 >= 1 && i0 <= nx - 2 && kr >= 2 && kr <= nz - 2) Fx[tx][0] = zflux_x(G, z, u, rhou, i0 - 1, kr);
-    } else if (ty == 1) {  // lower edge: the face above row k0 - 1
-        if (k0 >= 2 && k0 <= nz - 2 && i >= 1 && i <= nx - 2) Fz[0][tx] = zflux_z(G, z, w, rhow, i, k0 - 1);
+    if constexpr ((PARTS & PZ_ADV) != 0) {
+        // flux_u at i-1 = 0 is the periodic image of nx-2 (BoundaryProcess2D_center
+        // on flux_u): the ghost column 0 holds it for column 1
+        if (k >= 2 && k <= nz - 2 && i <= nx - 2) Fx[ty][tx + 1] = zflux_x(G, z, u, rhou, (i == 0) ? nx - 2 : i, k);
+        if (k >= 1 && k <= nz - 2 && i >= 1 && i <= nx - 2) Fz[ty + 1][tx] = zflux_z(G, z, w, rhow, i, k);
+        if (ty == 0 && tx < PZY) {  // west edge: the face east of column i0 - 1
+            const int kr = k0 + tx;
+            if (i0 >= 1 && i0 <= nx - 2 && kr >= 2 && kr <= nz - 2) Fx[tx][0] = zflux_x(G, z, u, rhou, i0 - 1, kr);
+        } else if (ty == 1) {  // lower edge: the face above row k0 - 1
+            if (k0 >= 2 && k0 <= nz - 2 && i >= 1 && i <= nx - 2) Fz[0][tx] = zflux_z(G, z, w, rhow, i, k0 - 1);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     if (i >= nx || k >= nz) return;
     double *zp = S.fut[F_ZETA];
     if (k == 0 || k == 1 || k == nz - 1) {
-        zp[ix(G, i, k)] = 0.;
+        if (PARTS == PZ_BOTH) zp[ix(G, i, k)] = 0.;
         return;
     }
     if (i < 1 || i > nx - 2) return;
     const size_t p = ix(G, i, k);
-    const double fu0 = Fx[ty][tx + 1], fum = Fx[ty][tx];
-    const double fw0 = Fz[ty + 1][tx], fwm = Fz[ty][tx];
-    const double px = (fu0 - fum) * G.r2dx / rhow[k];
-    const double pz = (fw0 - fwm) * G.r2dz / rhow[k];
-    const double dadv = -px - pz;
-    const double dold = ab2_hist(S.hist[H_ZETA_ADV], p, step, dadv);
-    double zeta_new = z[p] + 1.5 * G.dt * dadv - 0.5 * G.dt * dold;
+    double zeta_new;
+    if constexpr ((PARTS & PZ_ADV) != 0) {
+        const double fu0 = Fx[ty][tx + 1], fum = Fx[ty][tx];
+        const double fw0 = Fz[ty + 1][tx], fwm = Fz[ty][tx];
+        const double px = (fu0 - fum) * G.r2dx / rhow[k];
+        const double pz = (fw0 - fwm) * G.r2dz / rhow[k];
+        const double dadv = -px - pz;
+        const double dold = ab2_hist(S.hist[H_ZETA_ADV], p, step, dadv);
+        zeta_new = z[p] + 1.5 * G.dt * dadv - 0.5 * G.dt * dold;
+        if constexpr (PARTS == PZ_ADV) {
+            zp[p] = zeta_new;
+            return;
+        }
+    } else {
+        zeta_new = zp[p];
+    }
 
     const double *th = S.now[F_TH], *qv = S.now[F_QV], *qc = S.now[F_QC], *qr = S.now[F_QR], *thvb = S.P.thvb;
     const double gth = G.GRAVITY / rhow[k] * 0.5 *
@@ -174,7 +192,8 @@ __global__ void __launch_bounds__(PZX *PZY) k_pzeta(Grid G, StepPtrs S, int step
     db += -gqc - gqr;
     const double bold = ab2_hist(S.hist[H_TH_BUOY], p, step, db);
     zeta_new += (1.5 * G.dt * db) - (0.5 * G.dt * bold);
-    store_westdown(zp, G, i, k, zeta_new);
+    if constexpr (PARTS == PZ_BOTH) store_westdown(zp, G, i, k, zeta_new);
+    else zp[p] = zeta_new;
 }
 
 // ----------------------------------------------------------- scalars
@@ -367,8 +386,9 @@ __global__ void __launch_bounds__(T4X *T4Y) __attribute__((amdgpu_waves_per_eu(8
 // Second pass of NegativeValueProcess for qv, qc, qr (MicroPhysics.cpp:
 // 129-137), AddForcing (AddForcing.cpp:6-28: Q1LS/Q2LS and the random
 // theta kick) when `forcing`, and the center halos of thp, qvp, qcp, qrp.
+// halos == 0 is AddForcing alone (ratios < 0): interior thp and qvp only.
 __global__ void __launch_bounds__(BX *BY) k_scale_forcing(Grid G, StepPtrs S, int forcing, int tropical,
-                                                          int random_kick) {
+                                                          int random_kick, int halos) {
     const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
     if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
     const size_t p = ix(G, i, k);
@@ -379,7 +399,8 @@ __global__ void __launch_bounds__(BX *BY) k_scale_forcing(Grid G, StepPtrs S, in
         const double ratio = S.s[S_NEG_RATIO + q];
         if (ratio >= 0. && v > 0) v = v * ratio;
         if (forcing && tropical && f == F_QV) v += G.dt * S.P.Q2LS[k];
-        store_center(S.fut[f], G, i, k, v);
+        if (halos) store_center(S.fut[f], G, i, k, v);
+        else if (f == F_QV) S.fut[f][p] = v;
     }
     double th = S.fut[F_TH][p];
     if (forcing) {
@@ -390,7 +411,8 @@ __global__ void __launch_bounds__(BX *BY) k_scale_forcing(Grid G, StepPtrs S, in
         if (S.radcool) th += G.dt * (-2. / 86400.);
         if (random_kick) th += G.dt * S.forcing[p];
     }
-    store_center(S.fut[F_TH], G, i, k, th);
+    if (halos) store_center(S.fut[F_TH], G, i, k, th);
+    else S.fut[F_TH][p] = th;
 }
 
 // Halo value of BoundaryProcess2D_center at a perimeter cell.
@@ -430,6 +452,55 @@ __global__ void __launch_bounds__(BX *BY) k_bc_all(Grid G, StepPtrs S) {
     bc_center_cell(S.fut[F_QR], G, i, k);
 }
 
+// ------------------------------------------------ single operators alone
+// (the statics of class vvm one at a time; the timestep uses the fused
+// kernels above).  Each writes exactly what its reference function writes.
+
+// Advection_thermo (Advection.cpp:25-81) of field q (F_TH..F_QR): the
+// interior of its p-field and its AB2 history.
+__global__ void __launch_bounds__(BX *BY) k_adv_scalar(Grid G, StepPtrs S, int step, int q) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const int hid[NPROG] = {H_ZETA_ADV, H_TH_ADV, H_QV_ADV, H_QC_ADV, H_QR_ADV};
+    const double *f = S.now[q];
+    const size_t p = ix(G, i, k);
+    const double d = thermo_tend(G, f, S.u, S.w, S.P.rhou, S.P.rhow, i, k);
+    const double dold = ab2_hist(S.hist[hid[q]], p, step, d);
+    S.fut[q][p] = f[p] + 1.5 * G.dt * d - 0.5 * G.dt * dold;
+}
+
+// Advection_qrVT (Advection.cpp:153-192): the rain fall added to the
+// interior of qrp, and its AB2 history.
+__global__ void __launch_bounds__(BX *BY) k_qrvt(Grid G, StepPtrs S, int step) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const double *qr = S.now[F_QR], *rhow = S.P.rhow;
+    const size_t p = ix(G, i, k);
+    const double f1 = (k + 1 == G.nz - 1) ? 0. : vt_flux(G, qr, rhow, S.P.vtfac, i, k + 1);
+    const double f0 = vt_flux(G, qr, rhow, S.P.vtfac, i, k);
+    const double dvt = (f1 - f0) * G.r2dz / S.P.rhou[k];
+    const double vold = ab2_hist(S.hist[H_QR_VT], p, step, dvt);
+    S.fut[F_QR][p] += 1.5 * G.dt * dvt - 0.5 * G.dt * vold;
+}
+
+// Surface precipitation of pqr_pt (src/Iteration.cpp:114-119).
+__global__ void k_precip(Grid G, StepPtrs S) {
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > G.nx - 2) return;
+    const double r1 = S.P.rhou[1], q1 = S.now[F_QR][ix(G, i, 1)];
+    const double VT = 1E-2 * (3634 * pow(1E-3 * r1 * q1, 0.1346) * S.s[S_PRECIP_FAC]);
+    const double rain = -r1 * (0.5 * (S.w[ix(G, i, 2)] + 0.) - VT) * q1;
+    S.precip[i] = (rain < 0) ? 0. : rain;
+}
+
+// BoundaryProcess2D_center (westdown = 0) or _westdown on one field.
+__global__ void __launch_bounds__(BX *BY) k_bc_one(Grid G, double *a, int westdown) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i >= G.nx || k >= G.nz) return;
+    if (westdown) bc_westdown_cell(a, G, i, k);
+    else bc_center_cell(a, G, i, k);
+}
+
 // PoissonSolver::pubarTop_pt (src/PoissonSolver.cpp:655-676): domain-mean
 // vertical momentum flux and K_M shear at the top, AB2 for ubarTopp.
 __global__ void __launch_bounds__(1024) k_pubartop(Grid G, StepPtrs S, int step) {
@@ -467,7 +538,27 @@ inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY
 
 void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
     const dim3 g((G.nx + PZX - 1) / PZX, (G.nz + PZY - 1) / PZY);
-    hipLaunchKernelGGL(k_pzeta, g, dim3(PZX, PZY), 0, st, G, S, step);
+    hipLaunchKernelGGL(k_pzeta<PZ_BOTH>, g, dim3(PZX, PZY), 0, st, G, S, step);
+}
+void launch_advection_zeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
+    const dim3 g((G.nx + PZX - 1) / PZX, (G.nz + PZY - 1) / PZY);
+    hipLaunchKernelGGL(k_pzeta<PZ_ADV>, g, dim3(PZX, PZY), 0, st, G, S, step);
+}
+void launch_buoyancy(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
+    const dim3 g((G.nx + PZX - 1) / PZX, (G.nz + PZY - 1) / PZY);
+    hipLaunchKernelGGL(k_pzeta<PZ_BUOY>, g, dim3(PZX, PZY), 0, st, G, S, step);
+}
+void launch_advection_scalar(const Grid &G, const StepPtrs &S, int step, int field, hipStream_t st) {
+    hipLaunchKernelGGL(k_adv_scalar, grid2d(G), dim3(BX, BY), 0, st, G, S, step, field);
+}
+void launch_advection_qrvt(const Grid &G, const StepPtrs &S, int step, hipStream_t st) {
+    hipLaunchKernelGGL(k_qrvt, grid2d(G), dim3(BX, BY), 0, st, G, S, step);
+}
+void launch_precip(const Grid &G, const StepPtrs &S, hipStream_t st) {
+    hipLaunchKernelGGL(k_precip, dim3((G.nx + 255) / 256), dim3(256), 0, st, G, S);
+}
+void launch_bc_one(const Grid &G, double *a, bool westdown, hipStream_t st) {
+    hipLaunchKernelGGL(k_bc_one, grid2d(G), dim3(BX, BY), 0, st, G, a, int(westdown));
 }
 void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st) {
     const dim3 g((G.nx + T4X - 1) / T4X, (G.nz + T4Y - 1) / T4Y);
@@ -475,9 +566,9 @@ void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &r
     finalize<6>(rs, nblocks(g), S.s, RED_NEGVAL, S_NEG_RATIO, st);
 }
 void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
-                          hipStream_t st) {
+                          hipStream_t st, bool halos) {
     hipLaunchKernelGGL(k_scale_forcing, grid2d(G), dim3(BX, BY), 0, st, G, S, int(forcing), int(tropical),
-                       int(random_kick));
+                       int(random_kick), int(halos));
 }
 void launch_bc_uw(const Grid &G, const StepPtrs &S, hipStream_t st) {
     hipLaunchKernelGGL(k_bc_uw, grid2d(G), dim3(BX, BY), 0, st, G, S);

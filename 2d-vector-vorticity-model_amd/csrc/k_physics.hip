@@ -25,6 +25,12 @@ inline dim3 grid2d(const Grid &G) { return dim3((G.nx + BX - 1) / BX, (G.nz + BY
 // running the four in call order per point equals the reference's four
 // sweeps.  Then the first pass of NegativeValueProcess on qvp, qcp, qrp
 // (src/Iteration.cpp:309-311); k_finalize computes the three ratios.
+// PROC selects the processes (KP_*); the timestep runs them all with the
+// negative-value pass (KP_ALL).  One process alone (the reference's
+// MicroPhysics statics) writes only the fields and the rate it changes.
+enum { KP_AUTO = 1, KP_ACCR = 2, KP_EVAP = 4, KP_COND = 8, KP_NEG = 16, KP_ALL = 31 };
+
+template <int PROC>
 __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot rs) {
     const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
     const bool inside = (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2);
@@ -33,7 +39,7 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
         const size_t p = ix(G, i, k);
         const double dt = G.dt, pib = S.P.pib[k], rhou = S.P.rhou[k], pb = S.P.pb[k], pc = S.P.pcoef[k];
         double qc = S.fut[F_QC][p], qr = S.fut[F_QR][p], qv = S.fut[F_QV][p], th = S.fut[F_TH][p];
-        {  // autoconversion
+        if constexpr ((PROC & KP_AUTO) != 0) {  // autoconversion
             const double qcplus = maxs(0., qc);
             double ar = 0.001 * (qcplus - 0.001);
             ar = maxs(0., ar);
@@ -42,7 +48,7 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
             qr = qr + arcrdt;
             S.autoc[p] = arcrdt;
         }
-        {  // accretion
+        if constexpr ((PROC & KP_ACCR) != 0) {  // accretion
             const double qcplus = maxs(0., qc), qrplus = maxs(0., qr);
             const double cr = 2.2 * qcplus * (pow(qrplus, 0.875));
             const double arcrdt = mins(cr * dt, qcplus);
@@ -50,7 +56,7 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
             qr = qr + arcrdt;
             S.accr[p] = arcrdt;
         }
-        {  // rain evaporation
+        if constexpr ((PROC & KP_EVAP) != 0) {  // rain evaporation
             const double qrplus = maxs(0., qr), qvplus = maxs(0., qv);
             const double qvs = pc * exp(17.27 * (pib * th - 273.) / (pib * th - 36.));
             const double deficit = maxs((1. - qvplus / qvs), 0.);
@@ -63,7 +69,7 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
             th = th - G.Lv * erdt / (G.Cp * pib);
             S.evap[p] = erdt;
         }
-        {  // saturation adjustment
+        if constexpr ((PROC & KP_COND) != 0) {  // saturation adjustment
             const double qvs = pc * exp(17.27 * (pib * th - 273.) / (pib * th - 36.));
             const double tm = th * pib - 36.;
             const double phi = qvs * (17.27 * 237. * G.Lv) / (G.Cp * (tm * tm));
@@ -75,20 +81,51 @@ __global__ void __launch_bounds__(BX *BY) k_kessler(Grid G, StepPtrs S, RedSlot 
             S.cond[p] = C;
         }
         double q3[3] = {qv, qc, qr};
+        if constexpr ((PROC & KP_NEG) != 0) {
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (q3[q] >= 0.) part[2 * q] += q3[q];
-            else {
-                part[2 * q + 1] += q3[q];
-                q3[q] = 0.;
+            for (int q = 0; q < 3; q++) {
+                if (q3[q] >= 0.) part[2 * q] += q3[q];
+                else {
+                    part[2 * q + 1] += q3[q];
+                    q3[q] = 0.;
+                }
             }
         }
-        S.fut[F_QV][p] = q3[0];
-        S.fut[F_QC][p] = q3[1];
-        S.fut[F_QR][p] = q3[2];
-        S.fut[F_TH][p] = th;
+        constexpr bool wv = (PROC & (KP_EVAP | KP_COND | KP_NEG)) != 0;
+        constexpr bool wc = (PROC & (KP_AUTO | KP_ACCR | KP_COND | KP_NEG)) != 0;
+        constexpr bool wr = (PROC & (KP_AUTO | KP_ACCR | KP_EVAP | KP_NEG)) != 0;
+        constexpr bool wt = (PROC & (KP_EVAP | KP_COND)) != 0;
+        if (wv) S.fut[F_QV][p] = q3[0];
+        if (wc) S.fut[F_QC][p] = q3[1];
+        if (wr) S.fut[F_QR][p] = q3[2];
+        if (wt) S.fut[F_TH][p] = th;
     }
-    block_partials<6>(part, rs);
+    if constexpr ((PROC & KP_NEG) != 0) block_partials<6>(part, rs);
+}
+
+// MicroPhysics::NegativeValueProcess (MicroPhysics.cpp:116-138) on one
+// field: the first pass zeroes the negatives and sums positives and
+// negatives (k_finalize forms the ratio), the second rescales.
+__global__ void __launch_bounds__(BX *BY) k_negval_sums(Grid G, double *a, RedSlot rs) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    double part[2] = {0., 0.};
+    if (i >= 1 && i <= G.nx - 2 && k >= 1 && k <= G.nz - 2) {
+        const size_t p = ix(G, i, k);
+        const double v = a[p];
+        if (v >= 0.) part[0] += v;
+        else {
+            part[1] += v;
+            a[p] = 0.;
+        }
+    }
+    block_partials<2>(part, rs);
+}
+__global__ void __launch_bounds__(BX *BY) k_negval_scale(Grid G, double *a, const double *ratio) {
+    const int i = blockIdx.x * BX + threadIdx.x, k = blockIdx.y * BY + threadIdx.y;
+    if (i < 1 || i > G.nx - 2 || k < 1 || k > G.nz - 2) return;
+    const double r = *ratio;
+    const size_t p = ix(G, i, k);
+    if (r >= 0. && a[p] > 0) a[p] = a[p] * r;
 }
 
 // Turbulence::RKM_RKH coefficient part (Turbulent.cpp:13-45), with the
@@ -179,8 +216,22 @@ __global__ void __launch_bounds__(256) k_update_mean(Grid G, StepPtrs S, int shi
 }  // namespace
 
 void launch_kessler(const Grid &G, const StepPtrs &S, const RedSlot &rs, hipStream_t st) {
-    hipLaunchKernelGGL(k_kessler, grid2d(G), dim3(BX, BY), 0, st, G, S, rs);
+    hipLaunchKernelGGL(k_kessler<KP_ALL>, grid2d(G), dim3(BX, BY), 0, st, G, S, rs);
     finalize<6>(rs, nblocks(grid2d(G)), S.s, RED_NEGVAL, S_NEG_RATIO, st);
+}
+void launch_kessler_process(const Grid &G, const StepPtrs &S, int process, hipStream_t st) {
+    const RedSlot none{nullptr, 0};
+    switch (process) {
+        case 0: hipLaunchKernelGGL(k_kessler<KP_AUTO>, grid2d(G), dim3(BX, BY), 0, st, G, S, none); break;
+        case 1: hipLaunchKernelGGL(k_kessler<KP_ACCR>, grid2d(G), dim3(BX, BY), 0, st, G, S, none); break;
+        case 2: hipLaunchKernelGGL(k_kessler<KP_EVAP>, grid2d(G), dim3(BX, BY), 0, st, G, S, none); break;
+        default: hipLaunchKernelGGL(k_kessler<KP_COND>, grid2d(G), dim3(BX, BY), 0, st, G, S, none); break;
+    }
+}
+void launch_negative_value(const Grid &G, const StepPtrs &S, double *a, const RedSlot &rs, hipStream_t st) {
+    hipLaunchKernelGGL(k_negval_sums, grid2d(G), dim3(BX, BY), 0, st, G, a, rs);
+    finalize<2>(rs, nblocks(grid2d(G)), S.s, RED_NEGVAL, S_NEG_RATIO, st);
+    hipLaunchKernelGGL(k_negval_scale, grid2d(G), dim3(BX, BY), 0, st, G, a, S.s + S_NEG_RATIO);
 }
 void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st) {
     hipLaunchKernelGGL(k_turb_coeffs, grid2d(G), dim3(BX, BY), 0, st, G, S);

@@ -885,6 +885,94 @@ int vvm_dev_op(vvm_dev_t *h, int op) {
     return rc;
 }
 
+int vvm_dev_op_field(vvm_dev_t *h, int op, int field) {
+    if (!h) return VVM_ERR_ARG;
+    if (op < VVM_OP_ADVECTION_ZETA || op > VVM_OP_ADD_FORCING) return vvm_dev_op(h, op);
+    if (!h->uploaded) return fail(h, VVM_ERR_STATE, "op before upload");
+    HIPCHK(h, hipSetDevice(h->device));
+    const Grid &G = h->G;
+    const vvm_dev_config &c = h->cfg;
+    StepPtrs S = ptrs(h);
+    if (h->p_alias_now) {  // the operators read or write the p arrays: materialise them
+        const size_t bytes = size_t(G.nz) * G.pitch * sizeof(double);
+        for (int f = 0; f < NPROG; f++)
+            HIPCHK(h, hipMemcpyAsync(S.fut[f], S.now[f], bytes, hipMemcpyDeviceToDevice, h->st));
+        h->p_alias_now = false;
+    }
+    const bool scalar = field >= VVM_FIELD_TH && field <= VVM_FIELD_QR;
+    // reference field id -> device slot (VVM_FIELD_* and F_* share the order)
+    auto level_array = [&](int id) -> double * {
+        const int fld = id % 8, lev = id / 8;
+        if (fld < 0 || fld >= NPROG || lev < 0 || lev > 2) return nullptr;
+        return lev == VVM_LEVEL_M ? S.past[fld] : lev == VVM_LEVEL_NOW ? S.now[fld] : S.fut[fld];
+    };
+    const int step = h->step;
+    switch (op) {
+        case VVM_OP_ADVECTION_ZETA: launch_advection_zeta(G, S, step, h->st); break;
+        case VVM_OP_BOUYANCY: launch_buoyancy(G, S, step, h->st); break;
+        case VVM_OP_ADVECTION_THERMO:
+            if (!scalar) return fail(h, VVM_ERR_ARG, "Advection_thermo takes th, qv, qc or qr");
+            launch_advection_scalar(G, S, step, field, h->st);
+            break;
+        case VVM_OP_ADVECTION_QRVT: launch_advection_qrvt(G, S, step, h->st); break;
+        case VVM_OP_PTH:
+            launch_advection_scalar(G, S, step, F_TH, h->st);
+            launch_bc_one(G, S.fut[F_TH], false, h->st);
+            break;
+        case VVM_OP_PQV:
+        case VVM_OP_PQC: {
+            const int f = (op == VVM_OP_PQV) ? F_QV : F_QC;
+            launch_advection_scalar(G, S, step, f, h->st);
+            launch_negative_value(G, S, S.fut[f], h->red.misc, h->st);
+            launch_bc_one(G, S.fut[f], false, h->st);
+            break;
+        }
+        case VVM_OP_PQR:
+            launch_advection_scalar(G, S, step, F_QR, h->st);
+            launch_advection_qrvt(G, S, step, h->st);
+            launch_negative_value(G, S, S.fut[F_QR], h->red.misc, h->st);
+            launch_precip(G, S, h->st);
+            launch_bc_one(G, S.fut[F_QR], false, h->st);
+            break;
+        case VVM_OP_UPDATE_MEAN: launch_update_mean(G, S, false, h->st); break;
+        case VVM_OP_AUTOCONVERSION: launch_kessler_process(G, S, 0, h->st); break;
+        case VVM_OP_ACCRETION: launch_kessler_process(G, S, 1, h->st); break;
+        case VVM_OP_EVAPORATION: launch_kessler_process(G, S, 2, h->st); break;
+        case VVM_OP_CONDENSATION: launch_kessler_process(G, S, 3, h->st); break;
+        case VVM_OP_NEGATIVE_VALUE: {
+            double *a = level_array(field);
+            if (!a) return fail(h, VVM_ERR_ARG, "NegativeValueProcess: not a prognostic array");
+            launch_negative_value(G, S, a, h->red.misc, h->st);
+            break;
+        }
+        case VVM_OP_MPARAM: launch_diffusion(G, S, 0, h->st, F_ZETA); break;
+        case VVM_OP_HPARAM:
+            if (!scalar) return fail(h, VVM_ERR_ARG, "Hparam takes th, qv, qc or qr");
+            launch_diffusion(G, S, 0, h->st, field);
+            break;
+        case VVM_OP_NUDGE_THETA: launch_nudge(G, S, 1, h->st); break;
+        case VVM_OP_NUDGE_ZETA: launch_nudge(G, S, 2, h->st); break;
+        case VVM_OP_NUDGE_QV:
+            if (c.moisture_nudge_time != 0) launch_nudge_qv(G, S, c.moisture_nudge_time, h->st);
+            break;
+        case VVM_OP_ADD_FORCING: {
+            const double none[3] = {-1., -1., -1.};
+            HIPCHK(h, hipMemcpyAsync(h->s + S_NEG_RATIO, none, sizeof(none), hipMemcpyHostToDevice, h->st));
+            launch_scale_forcing(G, S, true, c.tropical_forcing != 0, h->status_for_adding_forcing != 0, h->st,
+                                 false);
+            break;
+        }
+        default: return fail(h, VVM_ERR_ARG, "unknown op");
+    }
+    int lrc = check_launch(h);
+    if (lrc) return lrc;
+    HIPCHK(h, hipStreamSynchronize(h->st));
+    double derr = 0.;
+    HIPCHK(h, hipMemcpy(&derr, h->s + S_DIFF_ERR, sizeof(double), hipMemcpyDeviceToHost));
+    if (derr != 0.) return fail(h, VVM_ERR_STATE, "diffusion sweep: a wavefront gave up waiting for its carry");
+    return VVM_OK;
+}
+
 static int poisson_common(vvm_dev *h, const double *d_b, double *d_x, vvm_solve_stats *stats) {
     const vvm_dev_config &c = h->cfg;
     h->pois.rhs_from_b(d_b, h->st);

@@ -72,7 +72,7 @@ struct Reductions {
 void launch_pzeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
 void launch_thermo4(const Grid &G, const StepPtrs &S, int step, const RedSlot &rs, hipStream_t st);
 void launch_scale_forcing(const Grid &G, const StepPtrs &S, bool forcing, bool tropical, bool random_kick,
-                          hipStream_t st);
+                          hipStream_t st, bool halos = true);
 void launch_bc_uw(const Grid &G, const StepPtrs &S, hipStream_t st);
 void launch_bc_all_prog(const Grid &G, const StepPtrs &S, hipStream_t st);
 // Whether a handle runs the microphysics phase on its second stream beside
@@ -81,13 +81,24 @@ void launch_bc_all_prog(const Grid &G, const StepPtrs &S, hipStream_t st);
 // block other members' work queued behind them.
 void dev_set_phase_overlap(vvm_dev_t *h, bool on);
 void launch_pubartop(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+// The reference operators one at a time (vvm_dev_op_field).
+void launch_advection_zeta(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+void launch_buoyancy(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+void launch_advection_scalar(const Grid &G, const StepPtrs &S, int step, int field, hipStream_t st);
+void launch_advection_qrvt(const Grid &G, const StepPtrs &S, int step, hipStream_t st);
+void launch_precip(const Grid &G, const StepPtrs &S, hipStream_t st);
+void launch_bc_one(const Grid &G, double *a, bool westdown, hipStream_t st);
+// process 0 autoconversion, 1 accretion, 2 evaporation, 3 condensation
+void launch_kessler_process(const Grid &G, const StepPtrs &S, int process, hipStream_t st);
+void launch_negative_value(const Grid &G, const StepPtrs &S, double *a, const RedSlot &rs, hipStream_t st);
 
 // Physics (k_physics.hip)
 void launch_kessler(const Grid &G, const StepPtrs &S, const RedSlot &rs, hipStream_t st);
 void launch_turbulence_coeffs(const Grid &G, const StepPtrs &S, hipStream_t st);
 bool diffusion_supported(const Grid &G);  // nx - 2 <= 8192
-// nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta into the sweep's store.
-void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st);
+// nudge bit 0 fuses Nudge_theta, bit 1 Nudge_zeta into the sweep's store;
+// only_field >= 0 sweeps that one field (Mparam or one Hparam) alone.
+void launch_diffusion(const Grid &G, const StepPtrs &S, int nudge, hipStream_t st, int only_field = -1);
 // Carry slots for the multi-workgroup sweep on `device` (allocated once per
 // process and device, never freed); nullptr when unavailable.
 unsigned long long *diffusion_slots(const Grid &G, int device);

@@ -10,10 +10,18 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
+#include <map>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <utility>
 #include <vector>
+
+#include "ReadConfig.hpp"
 
 Config_VVM::Config_VVM(double dt, double dx, double dz, int XRANGE, int ZRANGE, double TIMEEND, int TIMEROUTPUTSIZE,
                        std::string outputpath, int OUTPUTSTEP, double Kx, double Kz, double TIMETS, double tolerance,
@@ -45,13 +53,47 @@ void ensure_device(vvm &m) {
 }
 
 // One operator on the device with the host arrays current before and after.
-void run_op(vvm &m, int op, const char *name) {
+// The host state is uploaded first, so edits made to the host arrays between
+// two operator calls are seen, as the reference's operators see them.
+void run_op_field(vvm &m, int op, int field, const char *name) {
+    if (m.dev) m.dev_stale = true;
     ensure_device(m);
-    int rc = vvm_dev_op(m.dev, op);
+    int rc = vvm_dev_op_field(m.dev, op, field);
     if (rc > 0) std::printf("%s: the elliptic solve did not reach its stop rule\n", name);
     check(m, rc, name);
     m.host_stale = true;
     m.sync_to_host();
+}
+
+void run_op(vvm &m, int op, const char *name) { run_op_field(m, op, 0, name); }
+
+// Live models, so that the reference's NegativeValueProcess(double **, nx,
+// nz), which takes no model, can find the model (and device) that owns an
+// array.
+std::mutex g_models_mu;
+std::vector<vvm *> g_models;
+
+// VVM_FIELD_AT id of one of a model's fifteen prognostic arrays, or -1.
+int prognostic_id(const vvm &m, double **var) {
+    double **lev[3][5] = {{m.zetam, m.thm, m.qvm, m.qcm, m.qrm},
+                          {m.zeta, m.th, m.qv, m.qc, m.qr},
+                          {m.zetap, m.thp, m.qvp, m.qcp, m.qrp}};
+    for (int l = 0; l < 3; l++)
+        for (int f = 0; f < 5; f++)
+            if (lev[l][f] == var) return VVM_FIELD_AT(f, l);
+    return -1;
+}
+
+// Field id of a p-level array (zetap .. qrp), or -1.
+int future_field(const vvm &m, double **var) {
+    const int id = prognostic_id(m, var);
+    return (id >= 0 && id / 8 == VVM_LEVEL_P) ? id % 8 : -1;
+}
+
+[[noreturn]] void not_model_array(const char *what) {
+    std::fprintf(stderr, "vvm: %s: the array is not one of the model's prognostic arrays (the operators run on "
+                         "the device copy of the model state)\n", what);
+    std::exit(1);
 }
 
 bool is_output_step(const vvm &m, int step) {
@@ -93,9 +135,15 @@ vvm::vvm(const Config_VVM &c)
     dev_config.tol_rel = c.tol_rel;
     gpu_id = c.gpu_id;
     allocateMemory();
+    std::lock_guard<std::mutex> lk(g_models_mu);
+    g_models.push_back(this);
 }
 
 vvm::~vvm() {
+    {
+        std::lock_guard<std::mutex> lk(g_models_mu);
+        g_models.erase(std::remove(g_models.begin(), g_models.end(), this), g_models.end());
+    }
     if (dev) vvm_dev_destroy(dev);
     dev = nullptr;
     deallocateMemory();
@@ -263,7 +311,13 @@ void vvm::BoundaryProcess2D_westdown(double **var, int nx, int nz) {
 
 void vvm::BoundaryProcess2D_all(vvm &model) { run_op(model, VVM_OP_BC_ALL, "BoundaryProcess2D_all"); }
 
-void vvm::AddForcing(vvm &model) { run_op(model, VVM_OP_FORCING_BC, "AddForcing"); }
+void vvm::AddForcing(vvm &model) {
+    // the heating rate the reference's AddForcing reads (AddForcing.cpp:20)
+    ensure_device(model);
+    check(model, vvm_dev_set_radiation(model.dev, model.radiation_heating_ratecont, model.radiation_cooling ? 1 : 0),
+          "vvm_dev_set_radiation");
+    run_op(model, VVM_OP_ADD_FORCING, "AddForcing");
+}
 
 void vvm::PoissonSolver::InitPoissonMatrix(vvm &model) {
     if (model.dev) return;
@@ -286,6 +340,55 @@ void vvm::MicroPhysics::Kessler(vvm &model) { run_op(model, VVM_OP_MICROPHYSICS,
 void vvm::Iteration::pzeta_pt(vvm &model) { run_op(model, VVM_OP_PZETA, "pzeta_pt"); }
 void vvm::Iteration::pthermo_pt(vvm &model) { run_op(model, VVM_OP_PSCALARS, "pthermo_pt"); }
 void vvm::Iteration::nextTimeStep(vvm &model) { run_op(model, VVM_OP_NEXT, "nextTimeStep"); }
+void vvm::Iteration::pth_pt(vvm &model) { run_op(model, VVM_OP_PTH, "pth_pt"); }
+void vvm::Iteration::pqv_pt(vvm &model) { run_op(model, VVM_OP_PQV, "pqv_pt"); }
+void vvm::Iteration::pqc_pt(vvm &model) { run_op(model, VVM_OP_PQC, "pqc_pt"); }
+void vvm::Iteration::pqr_pt(vvm &model) { run_op(model, VVM_OP_PQR, "pqr_pt"); }
+void vvm::Iteration::updateMean(vvm &model) { run_op(model, VVM_OP_UPDATE_MEAN, "updateMean"); }
+
+void vvm::Advection_zeta(vvm &model) { run_op(model, VVM_OP_ADVECTION_ZETA, "Advection_zeta"); }
+void vvm::Bouyancy(vvm &model) { run_op(model, VVM_OP_BOUYANCY, "Bouyancy"); }
+void vvm::Advection_qrVT(vvm &model) { run_op(model, VVM_OP_ADVECTION_QRVT, "Advection_qrVT"); }
+void vvm::Advection_thermo(double **past, double **now, double **future, double ***dvar, vvm &model) {
+    // the reference's four calls (src/Iteration.cpp:37,91,101,110)
+    const int f = future_field(model, future);
+    double ***hist[5] = {nullptr, model.dth_advect, model.dqv_advect, model.dqc_advect, model.dqr_advect};
+    double **pasts[5] = {nullptr, model.thm, model.qvm, model.qcm, model.qrm};
+    double **nows[5] = {nullptr, model.th, model.qv, model.qc, model.qr};
+    if (f < VVM_FIELD_TH || past != pasts[f] || now != nows[f] || dvar != hist[f]) not_model_array("Advection_thermo");
+    run_op_field(model, VVM_OP_ADVECTION_THERMO, f, "Advection_thermo");
+}
+
+void vvm::Turbulence::Mparam(vvm &model, double **, double **var_future) {
+    if (var_future != model.zetap) not_model_array("Mparam");
+    run_op(model, VVM_OP_MPARAM, "Mparam");
+}
+void vvm::Turbulence::Hparam(vvm &model, double **, double **var_future) {
+    const int f = future_field(model, var_future);
+    if (f < VVM_FIELD_TH) not_model_array("Hparam");
+    run_op_field(model, VVM_OP_HPARAM, f, "Hparam");
+}
+void vvm::NumericalProcess::Nudge_theta(vvm &model) { run_op(model, VVM_OP_NUDGE_THETA, "Nudge_theta"); }
+void vvm::NumericalProcess::Nudge_zeta(vvm &model) { run_op(model, VVM_OP_NUDGE_ZETA, "Nudge_zeta"); }
+void vvm::NumericalProcess::Nudge_qv(vvm &model) { run_op(model, VVM_OP_NUDGE_QV, "Nudge_qv"); }
+void vvm::MicroPhysics::condensation(vvm &model) { run_op(model, VVM_OP_CONDENSATION, "condensation"); }
+void vvm::MicroPhysics::autoconversion(vvm &model) { run_op(model, VVM_OP_AUTOCONVERSION, "autoconversion"); }
+void vvm::MicroPhysics::accretion(vvm &model) { run_op(model, VVM_OP_ACCRETION, "accretion"); }
+void vvm::MicroPhysics::evaporation(vvm &model) { run_op(model, VVM_OP_EVAPORATION, "evaporation"); }
+void vvm::MicroPhysics::NegativeValueProcess(double **var, int nx, int nz) {
+    vvm *owner = nullptr;
+    int id = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_models_mu);
+        for (vvm *m : g_models)
+            if ((id = prognostic_id(*m, var)) >= 0) {
+                owner = m;
+                break;
+            }
+    }
+    if (!owner || owner->nx != nx || owner->nz != nz) not_model_array("NegativeValueProcess");
+    run_op_field(*owner, VVM_OP_NEGATIVE_VALUE, id, "NegativeValueProcess");
+}
 
 void vvm::Init::Init1d(vvm &model) {
     model.sync_to_host();
@@ -671,6 +774,35 @@ void vvm::Output::output_time_nc(int n, vvm &model) {
     }
 }
 
+void vvm::Output::create_directory(std::string path) {
+    // every prefix of the path, as mkdir -p
+    bool ok = !path.empty();
+    for (size_t pos = 1; ok && pos <= path.size(); pos++) {
+        if (pos < path.size() && path[pos] != '/') continue;
+        const std::string prefix = path.substr(0, pos);
+        if (mkdir(prefix.c_str(), 0755) != 0 && errno != EEXIST) ok = false;
+    }
+    if (!ok) std::cout << "Error on creating directory!\n" << std::endl;
+}
+
+void vvm::Output::printInit(vvm &model) {
+    std::cout << "z             Pbar          thb        thb_zeta     rhou       rhow       qvb   \t RH      pib"
+              << std::endl;
+    for (int k = 0; k <= model.nz - 1; k++) {
+        const double z = double(k - 0.5) * model.dz;
+        std::cout << std::fixed << std::setprecision(4) << z << "    " << model.pb[k] << "     " << model.thb[k] << "    "
+                  << model.thb_zeta[k] << "    " << model.rhou[k] << "     " << model.rhow[k] << "    "
+                  << model.qvb[k] * 1000 << "    " << model.qvb[k] / model.qvsb[k] << "    " << model.pib[k] << std::endl;
+    }
+    std::ofstream out(model.outputpath + "init.txt");
+    for (int k = 1; k <= model.nz - 2; k++) {
+        const double z = double(k - 0.5) * model.dz;
+        out << z << "    " << model.thb[k] << "    " << model.rhou[k] << "     " << model.rhow[k] << "   \t "
+            << model.qvb[k] << "    " << model.qvsb[k] << "    " << model.qvb[k] / model.qvsb[k] << "    "
+            << model.pib[k] << "    " << model.pb[k] << std::endl;
+    }
+}
+
 void vvm::Output::create_all_directory(vvm &model) {
     if (model.outputpath.empty()) return;
     std::string dirs[] = {model.outputpath, model.outputpath + "/state", model.outputpath + "/nc",
@@ -835,4 +967,26 @@ void vvm::Iteration::TimeMarching(vvm &model) {
     }
     if (model.print_steps) std::cout.flush();
     model.sync_to_host();
+}
+
+// key=value configuration file, '#' starting a comment, blanks around keys
+// and values dropped (the reference's vvm_read_config, src/ReadConfig.cpp).
+std::map<std::string, std::string> vvm_read_config(const std::string &filename) {
+    std::map<std::string, std::string> out;
+    std::ifstream in(filename);
+    std::string line;
+    auto trim = [](std::string t) {
+        const size_t a = t.find_first_not_of(" \t"), b = t.find_last_not_of(" \t");
+        return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+    };
+    while (std::getline(in, line)) {
+        const size_t hash = line.find('#');
+        if (hash != std::string::npos) line.erase(hash);
+        line = trim(line);
+        const size_t eq = line.find('=');
+        // a key needs an '=' with something after it ("KEY=" is skipped)
+        if (eq == std::string::npos || eq + 1 >= line.size()) continue;
+        out[trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+    }
+    return out;
 }

@@ -36,34 +36,18 @@
 #include <map>
 #include <string>
 
+#include "ReadConfig.hpp"
 #include "vvm.hpp"
 
 namespace {
 
-std::string trim(const std::string &s) {
-    size_t a = s.find_first_not_of(" \t\r\n");
-    if (a == std::string::npos) return "";
-    size_t b = s.find_last_not_of(" \t\r\n");
-    return s.substr(a, b - a + 1);
-}
 
 std::map<std::string, std::string> read_config(const std::string &path) {
-    std::map<std::string, std::string> m;
-    std::ifstream in(path);
-    if (!in) {
+    if (!std::ifstream(path)) {
         std::fprintf(stderr, "vvm_main: cannot open %s\n", path.c_str());
         std::exit(1);
     }
-    std::string line;
-    while (std::getline(in, line)) {
-        size_t hash = line.find('#');
-        if (hash != std::string::npos) line = line.substr(0, hash);
-        size_t eq = line.find('=');
-        if (eq == std::string::npos) continue;
-        std::string k = trim(line.substr(0, eq)), v = trim(line.substr(eq + 1));
-        if (!k.empty()) m[k] = v;
-    }
-    return m;
+    return vvm_read_config(path);
 }
 
 struct Keys {

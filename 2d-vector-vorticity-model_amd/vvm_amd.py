@@ -440,6 +440,11 @@ class Device:
     def op(self, name: str) -> int:
         return self._check(self.lib.vvm_dev_op(self.h, OPS[name]), f"op {name}")
 
+    def op_field(self, op: int, field: int = 0) -> int:
+        """One static of class vvm alone (VVM_OP_ADVECTION_ZETA ..
+        VVM_OP_NUDGE_QV), with the field argument it takes."""
+        return self._check(self.lib.vvm_dev_op_field(self.h, op, field), f"op {op}")
+
     def tune(self, steps: int = 3) -> "VvmTuneResult":
         r = VvmTuneResult()
         self._check(self.lib.vvm_dev_tune(self.h, steps, C.byref(r)), "tune")

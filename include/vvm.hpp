@@ -183,9 +183,21 @@ class vvm {
     // src/Boundary.cpp:34-52, on the device state.
     static void BoundaryProcess2D_all(vvm &model);
 
-    // Forcing of src/AddForcing.cpp:6-28 followed by BoundaryProcess2D_all,
-    // as called at src/Iteration.cpp:283-285.
+    // The forcing of src/AddForcing.cpp:6-28 (Q1LS/Q2LS when tropical_forcing
+    // is set, the radiation heating rate, the random theta kick while
+    // status_for_adding_forcing); the caller applies the halos, as
+    // src/Iteration.cpp:283-285 does.
     static void AddForcing(vvm &model);
+
+    // The reference's operators one at a time (src/Declare.hpp:725-749), each
+    // on the device with exactly the reference function's writes.  Array
+    // arguments must be the model's own arrays (the device holds their
+    // copies); any other array is an error.
+    static void Advection_zeta(vvm &model);                      // Advection.cpp:84-149
+    static void Advection_thermo(double **past, double **now, double **future, double ***dvar,
+                                 vvm &model);                     // Advection.cpp:25-81
+    static void Advection_qrVT(vvm &model);                      // Advection.cpp:153-192
+    static void Bouyancy(vvm &model);                            // Buoyancy.cpp:11-58
 
     class PoissonSolver {
        public:
@@ -202,6 +214,11 @@ class vvm {
         // K_M, K_H (src/Turbulent.cpp:5-66) and the in-place diffusion of
         // zeta, th, qv, qc, qr (Mparam / Hparam, src/Turbulent.cpp:68-114).
         static void RKM_RKH(vvm &model);
+        // The in-place diffusion of one field (var_future: zetap for Mparam,
+        // thp, qvp, qcp or qrp for Hparam; var_now is unused, as in the
+        // reference), src/Turbulent.cpp:68-114.
+        static void Mparam(vvm &model, double **var_now, double **var_future);
+        static void Hparam(vvm &model, double **var_now, double **var_future);
     };
 
     class NumericalProcess {
@@ -209,6 +226,9 @@ class vvm {
         // Nudge_theta, Nudge_zeta (unless CASE == 2) and Nudge_qv
         // (src/NumericalProcess.cpp:60-96, src/Iteration.cpp:415-417).
         static void Nudge(vvm &model);
+        static void Nudge_theta(vvm &model);  // src/NumericalProcess.cpp:60-71
+        static void Nudge_zeta(vvm &model);   // src/NumericalProcess.cpp:73-85
+        static void Nudge_qv(vvm &model);     // src/NumericalProcess.cpp:87-96
     };
 
     class MicroPhysics {
@@ -217,6 +237,13 @@ class vvm {
         // accretion, evaporation and the halos (src/Iteration.cpp:302-312,404)
         // in one device pass.
         static void Kessler(vvm &model);
+        static void condensation(vvm &model);    // src/MicroPhysics.cpp:7-26
+        static void autoconversion(vvm &model);  // src/MicroPhysics.cpp:29-51
+        static void accretion(vvm &model);       // src/MicroPhysics.cpp:54-77
+        static void evaporation(vvm &model);     // src/MicroPhysics.cpp:80-114
+        // src/MicroPhysics.cpp:116-138 on one of the fifteen prognostic
+        // arrays of a live model (found by its row pointers).
+        static void NegativeValueProcess(double **var, int nx, int nz);
     };
 
     class Init {
@@ -262,14 +289,25 @@ class vvm {
         static void output_state(int step, vvm &model);
         // The same record file at an explicit path.
         static void write_state(const std::string &path, vvm &model);
+        // The base state on stdout and <outputpath>/init.txt, in the
+        // reference's columns (src/Output.cpp:8-37); init2.txt, which lists
+        // the P3/RRTMGP level profiles this model does not carry, is not written.
+        static void printInit(vvm &model);
+        // mkdir -p of one directory (src/Output.cpp:329-338).
+        static void create_directory(std::string path);
     };
 
     class Iteration {
        public:
         static void pzeta_pt(vvm &model);
         // pth_pt, pqv_pt, pqc_pt and pqr_pt (src/Iteration.cpp:36-40,90-124)
-        // act on independent fields and run as one device pass.
+        // act on independent fields; pthermo_pt runs the four as one device pass.
         static void pthermo_pt(vvm &model);
+        static void pth_pt(vvm &model);
+        static void pqv_pt(vvm &model);
+        static void pqc_pt(vvm &model);
+        static void pqr_pt(vvm &model);
+        static void updateMean(vvm &model);  // src/Iteration.cpp:42-86
         static void TimeMarching(vvm &model);
         static void nextTimeStep(vvm &model);
     };

@@ -177,6 +177,43 @@ int vvm_dev_poisson_device(vvm_dev_t *h, const double *d_b, double *d_x, vvm_sol
 #define VVM_OP_BC_ALL 14      /* BoundaryProcess2D_all Boundary.cpp:34-52       */
 int vvm_dev_op(vvm_dev_t *h, int op);
 
+/* The reference's operators one at a time: each static of class vvm
+ * (src/Declare.hpp:707-935) runs alone on the device state and writes what
+ * the reference function writes, nothing more (the timestep fuses them).
+ * `field` names the array an operator takes as an argument in the
+ * reference; it is ignored by the others (pass 0). */
+#define VVM_FIELD_ZETA 0
+#define VVM_FIELD_TH 1
+#define VVM_FIELD_QV 2
+#define VVM_FIELD_QC 3
+#define VVM_FIELD_QR 4
+#define VVM_LEVEL_M 0        /* zetam, thm, ... */
+#define VVM_LEVEL_NOW 1      /* zeta, th, ...   */
+#define VVM_LEVEL_P 2        /* zetap, thp, ... */
+#define VVM_FIELD_AT(fld, level) ((level) * 8 + (fld))
+#define VVM_OP_ADVECTION_ZETA 20  /* vvm::Advection_zeta        Advection.cpp:84-149        */
+#define VVM_OP_BOUYANCY 21        /* vvm::Bouyancy              Buoyancy.cpp:11-58          */
+#define VVM_OP_ADVECTION_THERMO 22/* vvm::Advection_thermo      Advection.cpp:25-81; field TH..QR */
+#define VVM_OP_ADVECTION_QRVT 23  /* vvm::Advection_qrVT        Advection.cpp:153-192       */
+#define VVM_OP_PTH 24             /* Iteration::pth_pt          Iteration.cpp:36-40         */
+#define VVM_OP_PQV 25             /* Iteration::pqv_pt          Iteration.cpp:90-98         */
+#define VVM_OP_PQC 26             /* Iteration::pqc_pt          Iteration.cpp:100-107       */
+#define VVM_OP_PQR 27             /* Iteration::pqr_pt          Iteration.cpp:109-124       */
+#define VVM_OP_UPDATE_MEAN 28     /* Iteration::updateMean      Iteration.cpp:42-86         */
+#define VVM_OP_AUTOCONVERSION 29  /* MicroPhysics::autoconversion MicroPhysics.cpp:29-51    */
+#define VVM_OP_ACCRETION 30       /* MicroPhysics::accretion    MicroPhysics.cpp:54-77      */
+#define VVM_OP_EVAPORATION 31     /* MicroPhysics::evaporation  MicroPhysics.cpp:80-114     */
+#define VVM_OP_CONDENSATION 32    /* MicroPhysics::condensation MicroPhysics.cpp:7-26       */
+#define VVM_OP_NEGATIVE_VALUE 33  /* MicroPhysics::NegativeValueProcess MicroPhysics.cpp:116-138;
+                                     field VVM_FIELD_AT(fld, level), any prognostic array */
+#define VVM_OP_MPARAM 34          /* Turbulence::Mparam on zetap Turbulent.cpp:68-90      */
+#define VVM_OP_HPARAM 35          /* Turbulence::Hparam         Turbulent.cpp:92-114; field TH..QR (p level) */
+#define VVM_OP_NUDGE_THETA 36     /* NumericalProcess::Nudge_theta NumericalProcess.cpp:60-71 */
+#define VVM_OP_NUDGE_ZETA 37      /* NumericalProcess::Nudge_zeta  NumericalProcess.cpp:73-85 */
+#define VVM_OP_NUDGE_QV 38        /* NumericalProcess::Nudge_qv    NumericalProcess.cpp:87-96 */
+#define VVM_OP_ADD_FORCING 39     /* vvm::AddForcing alone      AddForcing.cpp:6-28         */
+int vvm_dev_op_field(vvm_dev_t *h, int op, int field);
+
 /* Measurement aid (no counterpart in the reference, whose only
  * instrumentation is the per-phase t_* timers): time every launch of one
  * kernel of vvm_dev_step with HIP events on the handle's stream.

@@ -67,6 +67,11 @@ def op(name: str, cfg, st, gsolve: int = GSOLVE_EXACT) -> None:
         _call(OPS[name], cfg, st)
 
 
+def op_single(op: int, cfg, st, field: int = 0) -> None:
+    """One static of class vvm alone, by its VVM_OP_* id."""
+    _call("vo_op_single", cfg, st, C.c_int(op), C.c_int(field))
+
+
 def set_radiation(rate, cooling: bool = False) -> None:
     """AddForcing's radiation heating rate (K/day, (nx, nz), kept alive by
     the caller) and RADIATIONCOOLING for later op/step calls; None clears."""

@@ -26,11 +26,18 @@ if [ ! -d "$REF/src" ]; then
     echo "build_ref: no reference sources at $REF; skipping" >&2
     exit 0
 fi
+# --stage-headers DIR: stage only the headers (Config.hpp with the edits
+# below, Declare.hpp, Timer.hpp) into DIR and stop; a test compiles code a
+# maintainer would add to the reference tree against them.
+HEADERS_ONLY=""
+if [ "${1:-}" = "--stage-headers" ]; then
+    HEADERS_ONLY=${2:?--stage-headers needs a directory}
+fi
 mkdir -p "$OUT"
 STAGE="$OUT/stage.$$"
 rm -rf "$STAGE"
 mkdir -p "$STAGE"
-trap 'rm -rf "$STAGE" "$STAGE".load "$STAGE".notrop' EXIT
+trap 'rm -rf "$STAGE" "$STAGE".load "$STAGE".notrop "$STAGE".dropin' EXIT
 
 FILES="Config.hpp Declare.hpp Timer.hpp Iteration.cpp Advection.cpp Buoyancy.cpp Boundary.cpp \
 PoissonSolver.cpp Turbulent.cpp MicroPhysics.cpp NumericalProcess.cpp AddForcing.cpp Init.cpp"
@@ -43,6 +50,13 @@ for flag in OUTPUTNC OUTPUTNCSAMEFILE P3_MICROPHY RTERRTMGP LOADFILE; do
 done
 grep -q '^// #define KESSLER_MICROPHY' "$cfg" || { echo "build_ref: KESSLER_MICROPHY line not found" >&2; exit 1; }
 sed -i 's|^// #define KESSLER_MICROPHY|#define KESSLER_MICROPHY|' "$cfg"
+
+if [ -n "$HEADERS_ONLY" ]; then
+    mkdir -p "$HEADERS_ONLY"
+    cp "$STAGE/Config.hpp" "$STAGE/Declare.hpp" "$STAGE/Timer.hpp" "$HEADERS_ONLY/"
+    echo "build_ref: staged headers in $HEADERS_ONLY"
+    exit 0
+fi
 
 af="$STAGE/AddForcing.cpp"
 [ "$(grep -c 'radiation_heating_rate' "$af")" = 1 ] || { echo "build_ref: AddForcing.cpp changed" >&2; exit 1; }
@@ -89,3 +103,22 @@ variant load "s|^// #define LOADFILE\b|#define LOADFILE|" \
     "0,/^    #define LOADINITPATH .*/s||    #define LOADINITPATH \"$REF/input/init.txt\"|"
 grep -q '^#define TROPICALFORCING\b' "$cfg" || { echo "build_ref: TROPICALFORCING line not found" >&2; exit 1; }
 variant notrop "s|^#define TROPICALFORCING\b|// #define TROPICALFORCING|"
+
+# The drop-in check: the reference's own src/Iteration.cpp (its TimeMarching
+# and p*_pt wrappers, unchanged) compiled against this build's include/
+# (Declare.hpp -> vvm.hpp) and linked with libvvm.so, so the reference's call
+# sequence drives the device-backed operator statics.  Built only when the
+# library exists; the binary goes to oracle/_ref/ with the others.
+PKG=$(cd "$HERE/../../2d-vector-vorticity-model_amd" && pwd)
+if [ -f "$PKG/libvvm.so" ]; then
+    D="$STAGE.dropin"
+    rm -rf "$D"
+    mkdir -p "$D"
+    cp "$REF/src/Iteration.cpp" "$D/"
+    $CXX -O2 -std=c++17 -w -DTROPICALFORCING -I"$HERE/../../include" -I"$HERE" \
+        "$D/Iteration.cpp" "$HERE/dropin_harness.cpp" -L"$PKG" -lvvm -lvvm_amd \
+        -Wl,-rpath,'$ORIGIN/../../2d-vector-vorticity-model_amd' -o "$OUT/vvm_dropin.tmp"
+    mv "$OUT/vvm_dropin.tmp" "$OUT/vvm_dropin"
+    rm -rf "$D"
+    echo "build_ref: built $OUT/vvm_dropin"
+fi

@@ -27,143 +27,8 @@
 #include <string>
 #include <vector>
 
-namespace {
+#include "state_io.hpp"
 
-struct Rec {
-    std::vector<int32_t> dims;
-    std::vector<double> data;
-};
-using RecMap = std::map<std::string, Rec>;
-
-RecMap read_state(const char *path) {
-    RecMap m;
-    std::ifstream f(path, std::ios::binary);
-    if (!f) { std::fprintf(stderr, "cannot open %s\n", path); std::exit(2); }
-    char magic[8];
-    f.read(magic, 8);
-    if (std::memcmp(magic, "VVMSTATE", 8) != 0) { std::fprintf(stderr, "bad magic\n"); std::exit(2); }
-    int32_t ver = 0, nrec = 0;
-    f.read(reinterpret_cast<char *>(&ver), 4);
-    f.read(reinterpret_cast<char *>(&nrec), 4);
-    for (int r = 0; r < nrec; ++r) {
-        char name[33] = {0};
-        int32_t ndim = 0, dims[3] = {1, 1, 1};
-        f.read(name, 32);
-        f.read(reinterpret_cast<char *>(&ndim), 4);
-        f.read(reinterpret_cast<char *>(dims), 12);
-        size_t n = size_t(dims[0]) * dims[1] * dims[2];
-        Rec rec;
-        rec.dims.assign(dims, dims + 3);
-        rec.data.resize(n);
-        f.read(reinterpret_cast<char *>(rec.data.data()), std::streamsize(n * 8));
-        if (!f) { std::fprintf(stderr, "truncated state file\n"); std::exit(2); }
-        m[name] = std::move(rec);
-    }
-    return m;
-}
-
-struct Writer {
-    std::vector<std::pair<std::string, Rec>> recs;
-    void add(const std::string &name, const double *p, int d0, int d1 = 1, int d2 = 1) {
-        Rec r;
-        r.dims = {d0, d1, d2};
-        r.data.assign(p, p + size_t(d0) * d1 * d2);
-        recs.emplace_back(name, std::move(r));
-    }
-    void scalar(const std::string &name, double v) { add(name, &v, 1); }
-    void save(const char *path) {
-        std::ofstream f(path, std::ios::binary);
-        f.write("VVMSTATE", 8);
-        int32_t ver = 1, nrec = int32_t(recs.size());
-        f.write(reinterpret_cast<char *>(&ver), 4);
-        f.write(reinterpret_cast<char *>(&nrec), 4);
-        for (auto &kv : recs) {
-            char name[32] = {0};
-            std::strncpy(name, kv.first.c_str(), 31);
-            int32_t ndim = 3;
-            f.write(name, 32);
-            f.write(reinterpret_cast<char *>(&ndim), 4);
-            f.write(reinterpret_cast<const char *>(kv.second.dims.data()), 12);
-            f.write(reinterpret_cast<const char *>(kv.second.data.data()),
-                    std::streamsize(kv.second.data.size() * 8));
-        }
-    }
-};
-
-double get1(const RecMap &m, const char *name, double dflt) {
-    auto it = m.find(name);
-    return it == m.end() ? dflt : it->second.data[0];
-}
-
-// Name -> storage of every array of the Kessler / AB2 build of class vvm.
-struct Binding {
-    const char *name;
-    double *ptr;
-    int d0, d1, d2;
-};
-
-std::vector<Binding> bindings(vvm &m) {
-    const int nx = m.nx, nz = m.nz;
-    return {
-        {"zeta", m.zetacont, nx, nz, 1}, {"zetam", m.zetamcont, nx, nz, 1}, {"zetap", m.zetapcont, nx, nz, 1},
-        {"th", m.thcont, nx, nz, 1},     {"thm", m.thmcont, nx, nz, 1},     {"thp", m.thpcont, nx, nz, 1},
-        {"qv", m.qvcont, nx, nz, 1},     {"qvm", m.qvmcont, nx, nz, 1},     {"qvp", m.qvpcont, nx, nz, 1},
-        {"qc", m.qccont, nx, nz, 1},     {"qcm", m.qcmcont, nx, nz, 1},     {"qcp", m.qcpcont, nx, nz, 1},
-        {"qr", m.qrcont, nx, nz, 1},     {"qrm", m.qrmcont, nx, nz, 1},     {"qrp", m.qrpcont, nx, nz, 1},
-        {"u", m.ucont, nx, nz, 1},       {"w", m.wcont, nx, nz, 1},
-        {"RKM", m.RKMcont, nx, nz, 1},   {"RKH", m.RKHcont, nx, nz, 1},
-        {"init_th_forcing", m.init_th_forcingcont, nx, nz, 1},
-        {"evaporation", m.evaporationcont, nx, nz, 1}, {"accretion", m.accretioncont, nx, nz, 1},
-        {"autoconversion", m.autoconversioncont, nx, nz, 1}, {"condensation", m.condensationcont, nx, nz, 1},
-        {"dth_advect", m.dth_advectcont, nx, nz, 2},     {"dth_buoyancy", m.dth_buoyancycont, nx, nz, 2},
-        {"dzeta_advect", m.dzeta_advectcont, nx, nz, 2}, {"dqv_advect", m.dqv_advectcont, nx, nz, 2},
-        {"dqc_advect", m.dqc_advectcont, nx, nz, 2},     {"dqr_advect", m.dqr_advectcont, nx, nz, 2},
-        {"dqr_VT", m.dqr_VTcont, nx, nz, 2},
-        {"precip", m.precip, nx, 1, 1}, {"xi", m.xi, nx, 1, 1}, {"uxi", m.uxi, nx, 1, 1},
-        {"rhou", m.rhou, nz, 1, 1},     {"rhow", m.rhow, nz, 1, 1},   {"pib", m.pib, nz, 1, 1},
-        {"pb", m.pb, nz, 1, 1},         {"thb", m.thb, nz, 1, 1},     {"thbm", m.thbm, nz, 1, 1},
-        {"thb_init", m.thb_init, nz, 1, 1}, {"thb_zeta", m.thb_zeta, nz, 1, 1},
-        {"thvb", m.thvb, nz, 1, 1},     {"thvbm", m.thvbm, nz, 1, 1}, {"qvb", m.qvb, nz, 1, 1},
-        {"qvb0", m.qvb0, nz, 1, 1},     {"qvsb", m.qvsb, nz, 1, 1},   {"z", m.z, nz, 1, 1},
-        {"z_zeta", m.z_zeta, nz, 1, 1}, {"lambda2", m.lambda2, nz, 1, 1},
-#if defined(TROPICALFORCING)
-        {"Q1LS", m.Q1LS, nz, 1, 1},     {"Q2LS", m.Q2LS, nz, 1, 1},
-#endif
-    };
-}
-
-void load_into(vvm &m, const RecMap &recs) {
-    for (auto &b : bindings(m)) {
-        auto it = recs.find(b.name);
-        if (it == recs.end()) continue;
-        size_t n = size_t(b.d0) * b.d1 * b.d2;
-        if (it->second.data.size() != n) {
-            std::fprintf(stderr, "size mismatch for %s\n", b.name);
-            std::exit(2);
-        }
-        std::memcpy(b.ptr, it->second.data.data(), n * 8);
-    }
-    m.ubarTop = get1(recs, "ubarTop", 0.);
-    m.ubarTopp = get1(recs, "ubarTopp", 0.);
-    m.ubarTopm = get1(recs, "ubarTopm", 0.);
-    m.dubarTop_advect[0] = get1(recs, "dubarTop_advect0", 0.);
-    m.dubarTop_advect[1] = get1(recs, "dubarTop_advect1", 0.);
-    m.step = int(get1(recs, "step", 0.));
-    m.status_for_adding_forcing = get1(recs, "status_for_adding_forcing", 0.) != 0.;
-}
-
-void save_from(vvm &m, Writer &w) {
-    for (auto &b : bindings(m)) w.add(b.name, b.ptr, b.d0, b.d1, b.d2);
-    w.scalar("ubarTop", m.ubarTop);
-    w.scalar("ubarTopp", m.ubarTopp);
-    w.scalar("ubarTopm", m.ubarTopm);
-    w.scalar("dubarTop_advect0", m.dubarTop_advect[0]);
-    w.scalar("dubarTop_advect1", m.dubarTop_advect[1]);
-    w.scalar("step", m.step);
-    w.scalar("status_for_adding_forcing", m.status_for_adding_forcing ? 1. : 0.);
-}
-
-}  // namespace
 
 int main(int argc, char **argv) {
     if (argc < 5) {
@@ -247,6 +112,27 @@ int main(int argc, char **argv) {
         else if (arg == "updateMean") vvm::Iteration::updateMean(model);
         else if (arg == "nextTimeStep") vvm::Iteration::nextTimeStep(model);
         else if (arg == "random_perturbation") vvm::Init::RandomPerturbation(model, model.step, -0.001, 0.001, 1.);
+        // the statics of class vvm one at a time (src/Declare.hpp:707-935)
+        else if (arg == "advection_zeta") model.Advection_zeta(model);
+        else if (arg == "bouyancy") model.Bouyancy(model);
+        else if (arg == "advection_thermo_th") model.Advection_thermo(model.thm, model.th, model.thp, model.dth_advect, model);
+        else if (arg == "advection_thermo_qv") model.Advection_thermo(model.qvm, model.qv, model.qvp, model.dqv_advect, model);
+        else if (arg == "advection_thermo_qc") model.Advection_thermo(model.qcm, model.qc, model.qcp, model.dqc_advect, model);
+        else if (arg == "advection_thermo_qr") model.Advection_thermo(model.qrm, model.qr, model.qrp, model.dqr_advect, model);
+        else if (arg == "advection_qrvt") model.Advection_qrVT(model);
+        else if (arg == "autoconversion") vvm::MicroPhysics::autoconversion(model);
+        else if (arg == "accretion") vvm::MicroPhysics::accretion(model);
+        else if (arg == "evaporation") vvm::MicroPhysics::evaporation(model);
+        else if (arg == "condensation") vvm::MicroPhysics::condensation(model);
+        else if (arg == "negval_qcp") vvm::MicroPhysics::NegativeValueProcess(model.qcp, model.nx, model.nz);
+        else if (arg == "negval_qc") vvm::MicroPhysics::NegativeValueProcess(model.qc, model.nx, model.nz);
+        else if (arg == "hparam_qv") vvm::Turbulence::Hparam(model, model.qv, model.qvp);
+        else if (arg == "hparam_qc") vvm::Turbulence::Hparam(model, model.qc, model.qcp);
+        else if (arg == "hparam_qr") vvm::Turbulence::Hparam(model, model.qr, model.qrp);
+        else if (arg == "nudge_theta") vvm::NumericalProcess::Nudge_theta(model);
+        else if (arg == "nudge_zeta") vvm::NumericalProcess::Nudge_zeta(model);
+        else if (arg == "nudge_qv") vvm::NumericalProcess::Nudge_qv(model);
+        else if (arg == "add_forcing") model.AddForcing(model);
         else { std::cout.rdbuf(old); std::fprintf(stderr, "unknown op %s\n", arg.c_str()); return 2; }
     } else if (mode == "solve") {
         // cal_w's right-hand side and solver (src/PoissonSolver.cpp:441-462)

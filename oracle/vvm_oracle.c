@@ -353,7 +353,15 @@ void vo_set_radiation(const double *rate, int cooling) {
  * surface fluxes) followed by BoundaryProcess2D_all (src/Iteration.cpp:
  * 283-285).  The radiation term (:20) and RADIATIONCOOLING (:21-23) apply
  * when vo_set_radiation gave them. */
+/* vvm::AddForcing alone (src/AddForcing.cpp:6-28). */
+static void add_forcing(const vvm_dev_config *c, vvm_host_view *v);
+
 int vo_forcing_bc(const vvm_dev_config *c, vvm_host_view *v) {
+    add_forcing(c, v);
+    return vo_bc_all(c, v);
+}
+
+static void add_forcing(const vvm_dev_config *c, vvm_host_view *v) {
     const int nx = c->nx, nz = c->nz;
     const double dt = c->dt;
     for (int k = 1; k <= nz - 2; k++) {
@@ -367,7 +375,6 @@ int vo_forcing_bc(const vvm_dev_config *c, vvm_host_view *v) {
             if (v->status_for_adding_forcing) AT(v->thp, i, k) += dt * AT(v->init_th_forcing, i, k);
         }
     }
-    return vo_bc_all(c, v);
 }
 
 /* ------------------------------------------------------------ elliptic */
@@ -639,11 +646,11 @@ int vo_cal_u(const vvm_dev_config *c, vvm_host_view *v, int gsolve) {
  * (src/MicroPhysics.cpp:29-51, 54-77, 80-114, 7-26) in the step's call order
  * (src/Iteration.cpp:303-306), then NegativeValueProcess on qvp, qcp, qrp
  * (:309-311) and BoundaryProcess2D_all (:404). */
-int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
+/* MicroPhysics::autoconversion (src/MicroPhysics.cpp:29-51). */
+static void autoconversion(const vvm_dev_config *c, vvm_host_view *v) {
     const int nx = c->nx, nz = c->nz;
     const double dt = c->dt;
-    double *qcp = v->qcp, *qrp = v->qrp, *qvp = v->qvp, *thp = v->thp;
-    const double *pib = v->pib, *rhou = v->rhou, *pb = v->pb;
+    double *qcp = v->qcp, *qrp = v->qrp;
     for (int k = 1; k <= nz - 2; k++) {
         for (int i = 1; i <= nx - 2; i++) {
             double qcplus = MAXS(0., AT(qcp, i, k));
@@ -655,6 +662,13 @@ int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
             AT(v->autoconversion, i, k) = arcrdt;
         }
     }
+}
+
+/* MicroPhysics::accretion (src/MicroPhysics.cpp:54-77). */
+static void accretion(const vvm_dev_config *c, vvm_host_view *v) {
+    const int nx = c->nx, nz = c->nz;
+    const double dt = c->dt;
+    double *qcp = v->qcp, *qrp = v->qrp;
     for (int k = 1; k <= nz - 2; k++) {
         for (int i = 1; i <= nx - 2; i++) {
             double qcplus = MAXS(0., AT(qcp, i, k));
@@ -666,6 +680,14 @@ int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
             AT(v->accretion, i, k) = arcrdt;
         }
     }
+}
+
+/* MicroPhysics::evaporation (src/MicroPhysics.cpp:80-114). */
+static void evaporation(const vvm_dev_config *c, vvm_host_view *v) {
+    const int nx = c->nx, nz = c->nz;
+    const double dt = c->dt;
+    double *qrp = v->qrp, *qvp = v->qvp, *thp = v->thp;
+    const double *pib = v->pib, *rhou = v->rhou, *pb = v->pb;
     for (int k = 1; k <= nz - 2; k++) {
         double pc = 380. / (pow(pib[k], c->Cp / c->Rd) * c->P0);
         for (int i = 1; i <= nx - 2; i++) {
@@ -683,6 +705,14 @@ int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
             AT(v->evaporation, i, k) = erdt;
         }
     }
+}
+
+/* MicroPhysics::condensation (src/MicroPhysics.cpp:7-26): saturation
+ * adjustment with Tetens' formula. */
+static void condensation(const vvm_dev_config *c, vvm_host_view *v) {
+    const int nx = c->nx, nz = c->nz;
+    double *qcp = v->qcp, *qvp = v->qvp, *thp = v->thp;
+    const double *pib = v->pib;
     for (int k = 1; k <= nz - 2; k++) {
         double pc = 380. / (pow(pib[k], c->Cp / c->Rd) * c->P0);
         for (int i = 1; i <= nx - 2; i++) {
@@ -696,9 +726,16 @@ int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
             AT(v->condensation, i, k) = C;
         }
     }
-    vo_negval(c, qvp);
-    vo_negval(c, qcp);
-    vo_negval(c, qrp);
+}
+
+int vo_microphysics(const vvm_dev_config *c, vvm_host_view *v) {
+    autoconversion(c, v);
+    accretion(c, v);
+    evaporation(c, v);
+    condensation(c, v);
+    vo_negval(c, v->qvp);
+    vo_negval(c, v->qcp);
+    vo_negval(c, v->qrp);
     return vo_bc_all(c, v);
 }
 
@@ -740,8 +777,9 @@ static void hparam(const vvm_dev_config *c, vvm_host_view *v, double *f) {
     }
 }
 
-/* Turbulence::RKM_RKH (src/Turbulent.cpp:5-66). */
-int vo_turbulence(const vvm_dev_config *c, vvm_host_view *v) {
+/* The coefficient part of Turbulence::RKM_RKH (src/Turbulent.cpp:5-45):
+ * K_M and K_H from the Richardson number, clamped, with center halos. */
+static int rkm_rkh_coeffs(const vvm_dev_config *c, vvm_host_view *v) {
     const grid_t g = mkgrid(c);
     const int nx = g.nx, nz = g.nz;
     const double *u = v->u, *w = v->w, *thp = v->thp, *l2 = v->lambda2;
@@ -774,6 +812,13 @@ int vo_turbulence(const vvm_dev_config *c, vvm_host_view *v) {
     }
     bc_center(v->RKM, nx, nz);
     bc_center(v->RKH, nx, nz);
+    return 0;
+}
+
+/* Turbulence::RKM_RKH (src/Turbulent.cpp:5-66): the coefficients, then
+ * Mparam on zetap and Hparam on thp, qvp, qcp, qrp (:47-52). */
+int vo_turbulence(const vvm_dev_config *c, vvm_host_view *v) {
+    rkm_rkh_coeffs(c, v);
     mparam(c, v, v->zetap);
     hparam(c, v, v->thp);
     hparam(c, v, v->qvp);
@@ -782,32 +827,46 @@ int vo_turbulence(const vvm_dev_config *c, vvm_host_view *v) {
     return 0;
 }
 
-/* NumericalProcess::Nudge_theta / Nudge_zeta / Nudge_qv
- * (src/NumericalProcess.cpp:60-96) as called at src/Iteration.cpp:415-417. */
-int vo_nudge(const vvm_dev_config *c, vvm_host_view *v) {
+/* NumericalProcess::Nudge_theta (src/NumericalProcess.cpp:60-71). */
+static void nudge_theta(const vvm_dev_config *c, vvm_host_view *v) {
     const int nx = c->nx, nz = c->nz;
-    const double dt = c->dt;
     const int ks = (int)(12000. / c->dz + 1);
     const int kr = (nz - 1) - ks;
     const double crad = 1. / 3600.;
-    if (kr >= 0) {
-        for (int k = ks; k < nz - 1; k++) {
-            double cgr = crad * (v->z[k] - v->z[kr]) / (v->z[nz - 1] - v->z[kr]);
-            for (int i = 1; i < nx - 1; i++)
-                AT(v->thp, i, k) -= dt * cgr * (AT(v->thp, i, k) - v->thb_init[k]);
-        }
-        if (c->CASE != 2) {
-            for (int k = ks; k < nz - 1; k++) {
-                double cgr = (1. / 360.) * (v->z_zeta[k] - v->z_zeta[kr]) / (v->z_zeta[nz - 1] - v->z_zeta[kr]);
-                for (int i = 1; i < nx - 1; i++) AT(v->zetap, i, k) -= dt * cgr * (AT(v->zetap, i, k));
-            }
-        }
+    if (kr < 0) return;
+    for (int k = ks; k < nz - 1; k++) {
+        double cgr = crad * (v->z[k] - v->z[kr]) / (v->z[nz - 1] - v->z[kr]);
+        for (int i = 1; i < nx - 1; i++) AT(v->thp, i, k) -= c->dt * cgr * (AT(v->thp, i, k) - v->thb_init[k]);
     }
-    if (c->moisture_nudge_time != 0) {
-        for (int i = 0; i <= nx - 1; i++)
-            for (int k = 0; k <= nz - 1; k++)
-                AT(v->qvp, i, k) = AT(v->qvp, i, k) + dt * (v->qvb0[k] - AT(v->qv, i, k)) / c->moisture_nudge_time;
+}
+
+/* NumericalProcess::Nudge_zeta (src/NumericalProcess.cpp:73-85). */
+static void nudge_zeta(const vvm_dev_config *c, vvm_host_view *v) {
+    const int nx = c->nx, nz = c->nz;
+    const int ks = (int)(12000. / c->dz + 1);
+    const int kr = (nz - 1) - ks;
+    if (kr < 0) return;
+    for (int k = ks; k < nz - 1; k++) {
+        double cgr = (1. / 360.) * (v->z_zeta[k] - v->z_zeta[kr]) / (v->z_zeta[nz - 1] - v->z_zeta[kr]);
+        for (int i = 1; i < nx - 1; i++) AT(v->zetap, i, k) -= c->dt * cgr * (AT(v->zetap, i, k));
     }
+}
+
+/* NumericalProcess::Nudge_qv (src/NumericalProcess.cpp:87-96): every point,
+ * ghosts included; off when moisture_nudge_time == 0. */
+static void nudge_qv(const vvm_dev_config *c, vvm_host_view *v) {
+    const int nx = c->nx, nz = c->nz;
+    if (c->moisture_nudge_time == 0) return;
+    for (int i = 0; i <= nx - 1; i++)
+        for (int k = 0; k <= nz - 1; k++)
+            AT(v->qvp, i, k) = AT(v->qvp, i, k) + c->dt * (v->qvb0[k] - AT(v->qv, i, k)) / c->moisture_nudge_time;
+}
+
+/* The three as called at src/Iteration.cpp:415-417. */
+int vo_nudge(const vvm_dev_config *c, vvm_host_view *v) {
+    nudge_theta(c, v);
+    if (c->CASE != 2) nudge_zeta(c, v);
+    nudge_qv(c, v);
     return 0;
 }
 
@@ -884,4 +943,59 @@ int vo_step(const vvm_dev_config *c, vvm_host_view *v, int nsteps, int gsolve) {
         vo_next_time_step(c, v);
     }
     return 0;
+}
+
+/* ------------------------------------------------------ single operators */
+
+/* The array a field id (VVM_FIELD_AT) names. */
+static double *field_array(vvm_host_view *v, int id) {
+    double *arr[3][5] = {{v->zetam, v->thm, v->qvm, v->qcm, v->qrm},
+                         {v->zeta, v->th, v->qv, v->qc, v->qr},
+                         {v->zetap, v->thp, v->qvp, v->qcp, v->qrp}};
+    const int fld = id % 8, lev = id / 8;
+    if (fld < 0 || fld > 4 || lev < 0 || lev > 2) return NULL;
+    return arr[lev][fld];
+}
+
+/* One static of class vvm (src/Declare.hpp:707-935) alone, by its VVM_OP_*
+ * id (include/vvm_dev.h); `field` as the device entry vvm_dev_op_field
+ * takes it.  Returns -1 for an unknown op or field. */
+int vo_op_single(const vvm_dev_config *c, vvm_host_view *v, int op, int field) {
+    double *now[5] = {v->zeta, v->th, v->qv, v->qc, v->qr};
+    double *fut[5] = {v->zetap, v->thp, v->qvp, v->qcp, v->qrp};
+    double *hist[5] = {v->dzeta_advect, v->dth_advect, v->dqv_advect, v->dqc_advect, v->dqr_advect};
+    const int scalar = (field >= VVM_FIELD_TH && field <= VVM_FIELD_QR);
+    switch (op) {
+        case VVM_OP_ADVECTION_ZETA: adv_zeta(c, v); return 0;
+        case VVM_OP_BOUYANCY: buoyancy(c, v); return 0;
+        case VVM_OP_ADVECTION_THERMO:
+            if (!scalar) return -1;
+            adv_thermo(c, v, now[field], fut[field], hist[field]);
+            return 0;
+        case VVM_OP_ADVECTION_QRVT: adv_qrvt(c, v); return 0;
+        case VVM_OP_PTH: return vo_pth_pt(c, v);
+        case VVM_OP_PQV: return vo_pqv_pt(c, v);
+        case VVM_OP_PQC: return vo_pqc_pt(c, v);
+        case VVM_OP_PQR: return vo_pqr_pt(c, v);
+        case VVM_OP_UPDATE_MEAN: return vo_update_mean(c, v);
+        case VVM_OP_AUTOCONVERSION: autoconversion(c, v); return 0;
+        case VVM_OP_ACCRETION: accretion(c, v); return 0;
+        case VVM_OP_EVAPORATION: evaporation(c, v); return 0;
+        case VVM_OP_CONDENSATION: condensation(c, v); return 0;
+        case VVM_OP_NEGATIVE_VALUE: {
+            double *a = field_array(v, field);
+            if (!a) return -1;
+            return vo_negval(c, a);
+        }
+        case VVM_OP_MPARAM: mparam(c, v, v->zetap); return 0;
+        case VVM_OP_HPARAM:
+            if (!scalar) return -1;
+            hparam(c, v, fut[field]);
+            return 0;
+        case VVM_OP_NUDGE_THETA: nudge_theta(c, v); return 0;
+        case VVM_OP_NUDGE_ZETA: nudge_zeta(c, v); return 0;
+        case VVM_OP_NUDGE_QV: nudge_qv(c, v); return 0;
+        case VVM_OP_ADD_FORCING: add_forcing(c, v); return 0;
+        default: return -1;
+    }
 }

@@ -36,6 +36,8 @@ int vo_turbulence(const vvm_dev_config *c, vvm_host_view *v);
 int vo_nudge(const vvm_dev_config *c, vvm_host_view *v);
 int vo_update_mean(const vvm_dev_config *c, vvm_host_view *v);
 int vo_next_time_step(const vvm_dev_config *c, vvm_host_view *v);
+/* One static of class vvm alone (VVM_OP_ADVECTION_ZETA .. VVM_OP_ADD_FORCING). */
+int vo_op_single(const vvm_dev_config *c, vvm_host_view *v, int op, int field);
 
 /* Init::RandomPerturbation (src/Init.cpp:488-515): init_th_forcing from
  * std::mt19937(t) and std::normal_distribution (vvm_oracle_rng.cpp). */

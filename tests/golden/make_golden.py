@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path[:0] = [ROOT, os.path.join(ROOT, "2d-vector-vorticity-model_amd"), os.path.join(ROOT, "tests")]
 
 import vvm_amd as V  # noqa: E402
-from helpers import branch_cases, input_checksums, perturbed_state  # noqa: E402
+from helpers import branch_cases, input_checksums, perturbed_state, single_case  # noqa: E402
 from oracle import oracle as O  # noqa: E402
 
 REF_OPS = ["pzeta_pt", "pth_pt", "pqv_pt", "pqc_pt", "pqr_pt", "forcing_bc", "bc_all", "pubarTop_pt", "cal_w",
@@ -64,6 +64,34 @@ def ops_fixture(tmp):
                         continue
                     data[f"s{step}/{op}/{n}"] = a
     np.savez_compressed(os.path.join(HERE, "ops_tiny.npz"), **data)
+
+
+# The statics of class vvm one at a time: reference harness op -> (VVM_OP_*
+# id, field argument) as vvm_dev_op_field / vo_op_single take them.
+SINGLE_OPS = {
+    "advection_zeta": (20, 0), "bouyancy": (21, 0),
+    "advection_thermo_th": (22, 1), "advection_thermo_qv": (22, 2), "advection_thermo_qc": (22, 3),
+    "advection_thermo_qr": (22, 4), "advection_qrvt": (23, 0),
+    "pth_pt": (24, 0), "pqv_pt": (25, 0), "pqc_pt": (26, 0), "pqr_pt": (27, 0), "updateMean": (28, 0),
+    "autoconversion": (29, 0), "accretion": (30, 0), "evaporation": (31, 0), "condensation": (32, 0),
+    "negval_qvp": (33, 2 * 8 + 2), "negval_qcp": (33, 2 * 8 + 3), "negval_qc": (33, 1 * 8 + 3),
+    "mparam": (34, 0), "hparam_th": (35, 1), "hparam_qv": (35, 2), "hparam_qc": (35, 3), "hparam_qr": (35, 4),
+    "nudge_theta": (36, 0), "nudge_zeta": (37, 0), "nudge_qv": (38, 0), "add_forcing": (39, 0),
+}
+
+
+def single_fixture(tmp):
+    c, st = single_case()
+    data = {"in_checksum": input_checksums(st)}
+    for op in SINGLE_OPS:
+        r = run_ref("op", op, st, c.cfg, tmp)
+        for n, a in r.items():
+            if n in st.a or n in SCAL:
+                ref_in = st.a.get(n)
+                if ref_in is not None and ref_in.size == a.size and np.array_equal(ref_in.reshape(a.shape), a):
+                    continue
+                data[f"{op}/{n}"] = a
+    np.savez_compressed(os.path.join(HERE, "ops_single.npz"), **data)
 
 
 def step_fixture(tmp):
@@ -201,7 +229,7 @@ def main():
         subprocess.run(["bash", os.path.join(ROOT, "oracle", "ref", "build_ref.sh")], check=True)
     if not O.ref_available():
         sys.exit("reference binary unavailable")
-    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long", "init", "branches"}
+    which = set(sys.argv[1:]) or {"ops", "steps", "misc", "long", "init", "branches", "single"}
     with tempfile.TemporaryDirectory(dir=os.path.join(ROOT, "oracle", "_ref")) as tmp:
         if "ops" in which:
             ops_fixture(tmp)
@@ -215,6 +243,8 @@ def main():
             init_fixture(tmp)
         if "branches" in which:
             branch_fixture(tmp)
+        if "single" in which:
+            single_fixture(tmp)
 
 
 if __name__ == "__main__":

@@ -126,3 +126,18 @@ def input_checksums(st: V.HostState) -> np.ndarray:
     """Per-array sums of |value| of a state, in sorted name order: the
     tests regenerate each initial state and compare these first."""
     return np.array([float(np.abs(st.a[n]).sum()) for n in sorted(st.a)])
+
+
+def single_case():
+    """A perturbed state on a grid tall enough for the nudged layer (z > 12
+    km), with p-fields near the now-fields (some cloud negative) and qv
+    nudging on: every single operator changes something."""
+    c = V.grid_case(24, 64, dx=200.0, dt=2.0, moisture_nudge_time=3600.0)
+    st = perturbed_state(c, seed=77, step=5)
+    rng = np.random.default_rng(78)
+    for f in ("zeta", "th", "qv", "qc", "qr"):
+        st[f + "p"][:] = st[f] * (1 + rng.normal(0, 1e-3, st[f].shape))
+    st["qcp"][:] -= 2e-5
+    st["qc"][:] -= 1e-4
+    st["qvp"][3:7, 10:12] = -1e-5  # a few negatives for NegativeValueProcess
+    return c, st

@@ -322,3 +322,41 @@ def test_host_microphysics_coupling(oracle):
         dev.download(got)
     assert calls == [st.step + 1, st.step + 2], calls
     assert_fields_close(got, want, COMPARE_STEP, 1e-10, where="c0 2 steps, host microphysics")
+
+
+def _single_ops():
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(here, "golden", "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.SINGLE_OPS
+
+
+SINGLE = _single_ops()
+
+
+@pytest.mark.parametrize("name", list(SINGLE))
+def test_single_operator_parity(oracle, name):
+    """Each static of class vvm alone (vvm_dev_op_field) against the oracle's
+    (pinned bit for bit to the compiled reference's own function): the
+    arrays it changes to 1e-12, every other array untouched."""
+    from helpers import single_case
+    c, st = single_case()
+    op, field = SINGLE[name]
+    want = st.copy()
+    oracle.op_single(op, c.cfg, want, field)
+    with V.Device(c.cfg) as dev:
+        dev.upload(st)
+        dev.op_field(op, field)
+        got = st.copy()
+        dev.download(got)
+    for f in st.a:
+        if f in ("thb_zeta", "init_th_forcing"):  # formed on download / not state
+            continue
+        if np.array_equal(want[f], st[f]):
+            assert np.array_equal(got[f], st[f]), f"{name} changed {f}"
+        else:
+            e = rel_err(got[f], want[f])
+            assert e <= 1e-12, f"{name}/{f}: rel {e:.3e}"

@@ -176,3 +176,44 @@ def test_branch_fixture_discriminates(oracle, branch_gold, branch_states, name, 
     field = {"addforcingtime": "th", "moisture_nudge_time": "qv", "CASE": "zeta"}[flip]
     want = branch_gold[f"{name}/{n}/{field}"].reshape(st[field].shape)
     assert rel_err(st[field], want) > 1e-9
+
+
+@pytest.fixture(scope="module")
+def single_gold():
+    return np.load(os.path.join(GOLD, "ops_single.npz"))
+
+
+def single_ops():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(GOLD, "make_golden.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod.SINGLE_OPS
+
+
+SINGLE = single_ops()
+
+
+@pytest.mark.parametrize("name", list(SINGLE))
+def test_single_operator_vs_reference(oracle, single_gold, name):
+    """Each static of class vvm alone (src/Declare.hpp:707-935), as the
+    per-operator C++ API exposes it, against the compiled reference's own
+    function on the same state: bit for bit."""
+    from helpers import input_checksums, single_case
+    c, st = single_case()
+    np.testing.assert_allclose(input_checksums(st), single_gold["in_checksum"], rtol=1e-15, atol=0)
+    op, field = SINGLE[name]
+    oracle.op_single(op, c.cfg, st, field)
+    keys = [k for k in single_gold.files if k.startswith(name + "/")]
+    arrays = [k for k in keys if k.split("/")[1] not in SCAL]
+    assert arrays, "the reference changed nothing"
+    for k in arrays:
+        f = k.split("/")[1]
+        want = single_gold[k].reshape(st[f].shape)
+        assert rel_err(st[f], want) == 0.0, f"{name}/{f}: rel {rel_err(st[f], want):.3e}"
+    # and nothing else: every array the reference left alone is unchanged
+    _, st0 = single_case()
+    changed = {k.split("/")[1] for k in arrays}
+    for f in st.a:
+        if f not in changed:
+            assert np.array_equal(st[f], st0[f]), f"{name} changed {f}"
