@@ -74,10 +74,24 @@ def rel_err(a: np.ndarray, b: np.ndarray) -> float:
     return d / nb
 
 
+# Diagnostics judged against the field they feed rather than their own norm:
+# uxi (the mean-free x-derivative of the top-row G solution) enters u additively
+# as u_top = uxi + ubarTopp (src/PoissonSolver.cpp:620-640).  It is formed from
+# w one row below the lid, where w is small, so a round-off of the elliptic
+# solve at the size of w in the interior is a large part of uxi when the flow
+# near the lid is weak (a quiet start: ||uxi|| ~ 3e-4 ||u||).
+SCALE_OF = {"uxi": "u"}
+
+
 def assert_fields_close(got: V.HostState, want: V.HostState, names, tol: float, where: str = ""):
     bad = []
     for n in names:
-        e = rel_err(got[n], want[n])
+        if n in SCALE_OF:
+            ref = max(float(np.linalg.norm(want[n])), float(np.linalg.norm(want[SCALE_OF[n]])))
+            d = float(np.linalg.norm(np.asarray(got[n]) - np.asarray(want[n])))
+            e = d / ref if ref > 0 else (0.0 if d < 1e-300 else float("inf"))
+        else:
+            e = rel_err(got[n], want[n])
         if not (e <= tol):
             bad.append(f"{n}: rel {e:.3e}")
     assert not bad, f"{where} fields above {tol:g}: " + ", ".join(bad)
