@@ -9,6 +9,9 @@
 
 #include "vvm_internal.h"
 
+#include <cstdio>
+#include <mutex>
+
 namespace vvmd {
 
 namespace {
@@ -259,3 +262,83 @@ void launch_update_mean(const Grid &G, const StepPtrs &S, bool shift_ubar, hipSt
 }
 
 }  // namespace vvmd
+
+// ------------------------------------------------ no-allocation entry point
+//
+// vvm_negative_value_noalloc runs the kernels of the handle's
+// NegativeValueProcess operator (k_negval_sums, k_finalize, k_negval_scale)
+// on a caller's device field.  Their scratch -- the two partial sums of each
+// workgroup, then the correction ratio -- is the file-scope array below
+// instead of a handle's buffers, so the call allocates nothing.  Every
+// workgroup overwrites its own two partials and k_finalize reads only this
+// call's workgroups and writes the ratio before k_negval_scale reads it, so
+// nothing needs clearing between calls.  Calls that overlap on one GPU share
+// the array and race.
+
+namespace vvmd {
+constexpr int kNoAllocMaxBlocks = 8192;  // 64x4-point workgroups: 2050x992 points at most
+}  // namespace vvmd
+
+static __device__ __align__(256) double scratch[2 * vvmd::kNoAllocMaxBlocks + 32];
+
+namespace {
+// Device address of `scratch` on each device, taken once with that device
+// current (each GPU holds its own copy of the array at its own address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return VVM_ERR_HIP;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "vvm_negative_value_noalloc: hipGetSymbolAddress(scratch) on device %d: %s\n", dev,
+                    hipGetErrorString(e));
+            return VVM_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return VVM_OK;
+}
+}  // namespace
+
+extern "C" int vvm_negative_value_noalloc(double *d_var, int nx, int nz, int pitch, void *stream) {
+    using namespace vvmd;
+    if (!d_var || nx < 3 || nz < 3 || pitch < nx) {
+        fprintf(stderr, "vvm_negative_value_noalloc: bad arguments (var %p, nx %d, nz %d, pitch %d)\n",
+                static_cast<void *>(d_var), nx, nz, pitch);
+        return VVM_ERR_ARG;
+    }
+    Grid G{};
+    G.nx = nx;
+    G.nz = nz;
+    G.pitch = pitch;
+    const dim3 g = grid2d(G);
+    const int nb = nblocks(g);
+    if (nb > kNoAllocMaxBlocks) {
+        fprintf(stderr,
+                "vvm_negative_value_noalloc: a %d x %d field needs %d workgroups of %dx%d points; the "
+                "no-allocation scratch holds %d (use vvm_dev_op_field for larger grids)\n",
+                nx, nz, nb, BX, BY, kNoAllocMaxBlocks);
+        return VVM_ERR_SIZE;
+    }
+    double *s = nullptr;
+    if (const int rc = scratch_for_current_device(&s)) return rc;
+    const RedSlot rs{s, kNoAllocMaxBlocks};
+    double *ratio = s + 2 * kNoAllocMaxBlocks;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_negval_sums, g, dim3(BX, BY), 0, st, G, d_var, rs);
+    finalize<2>(rs, nb, ratio, RED_NEGVAL, 0, st);
+    hipLaunchKernelGGL(k_negval_scale, g, dim3(BX, BY), 0, st, G, d_var, ratio);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        fprintf(stderr, "vvm_negative_value_noalloc: launch failed: %s\n", hipGetErrorString(e));
+        return VVM_ERR_HIP;
+    }
+    return VVM_OK;
+}

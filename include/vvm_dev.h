@@ -214,6 +214,23 @@ int vvm_dev_op(vvm_dev_t *h, int op);
 #define VVM_OP_ADD_FORCING 39     /* vvm::AddForcing alone      AddForcing.cpp:6-28         */
 int vvm_dev_op_field(vvm_dev_t *h, int op, int field);
 
+/* NegativeValueProcess on a caller's device field, allocating nothing.  No
+ * counterpart in the reference: this is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It runs the kernels of
+ * vvm_dev_op_field(h, VVM_OP_NEGATIVE_VALUE, f) on d_var, whose element
+ * (i,k) of the reference's var[i][k] is at d_var[k*pitch + i] (ghost cells
+ * included, pitch >= nx; only i = 1..nx-2, k = 1..nz-2 are read or written),
+ * with their per-workgroup partial sums and the correction ratio in a
+ * static device array of 128 KiB that each GPU holds until the process
+ * ends.  Limit: 8192 workgroups of 64x4 points, i.e. ceil(nx/64)*ceil(nz/4)
+ * <= 8192 (2050 x 992 points); larger grids return VVM_ERR_SIZE with a
+ * message on stderr.  Asynchronous on `stream` (a hipStream_t; NULL is the
+ * null stream): the caller synchronises.  No hipMalloc/hipFree, no
+ * allocating library call.  Calls that overlap on one GPU share the array
+ * and race; serialise them (one stream, or events). */
+int vvm_negative_value_noalloc(double *d_var, int nx, int nz, int pitch, void *stream);
+
 /* Measurement aid (no counterpart in the reference, whose only
  * instrumentation is the per-phase t_* timers): time every launch of one
  * kernel of vvm_dev_step with HIP events on the handle's stream.
