@@ -3,7 +3,8 @@
 //
 // Features on the unknown grid (rows 1..M, N periodic columns), each scaled
 // by its RMS over the grid:
-//   0: r = f - L xe   (residual of the extrapolated guess xe = 2 x1 - x2)
+//   0: r = f - L xe   (residual of the extrapolated guess xe: 3 (x1 - x2) + x3
+//                      once three solutions exist, 2 x1 - x2 before)
 //   1: d = x1 - x2    (last solution increment)
 //   2: xe
 // The network predicts (x - xe) / s_d with s_d = RMS(d); the guess is
@@ -22,7 +23,9 @@ namespace {
 constexpr int GX = 64, GY = 4;
 inline dim3 ggrid(int N, int M) { return dim3((N + GX - 1) / GX, (M + GY - 1) / GY); }
 
-__device__ inline double xe_at(const double *x1, const double *x2, size_t a) { return 2. * x1[a] - x2[a]; }
+__device__ inline double xe_at(const double *x1, const double *x2, const double *x3, int ord, size_t a) {
+    return extrap_at(x1, x2, x3, ord, a);
+}
 
 // Partials: the three features' sums of squares, and ||wD r||^2 of the
 // extrapolated guess (the solver's own norm, wD = rhow: the reference
@@ -30,6 +33,7 @@ __device__ inline double xe_at(const double *x1, const double *x2, size_t a) { r
 __global__ void __launch_bounds__(GX *GY) k_guess_features(int N, int M, const double *__restrict__ f,
                                                            const double *__restrict__ x1,
                                                            const double *__restrict__ x2,
+                                                           const double *__restrict__ x3, int ord,
                                                            const double *__restrict__ cx,
                                                            const double *__restrict__ cu,
                                                            const double *__restrict__ wD, float *__restrict__ feat,
@@ -39,9 +43,9 @@ __global__ void __launch_bounds__(GX *GY) k_guess_features(int N, int M, const d
     if (i < N && j <= M) {
         const int im = (i == 0) ? N - 1 : i - 1, ip = (i == N - 1) ? 0 : i + 1;
         const size_t r = size_t(j) * N;
-        const double c = xe_at(x1, x2, r + i);
-        const double Lx = cx[j] * (2. * c - xe_at(x1, x2, r + im) - xe_at(x1, x2, r + ip)) +
-                          cu[j] * (c - xe_at(x1, x2, r + N + i)) + cu[j - 1] * (c - xe_at(x1, x2, r - N + i));
+        const double c = xe_at(x1, x2, x3, ord, r + i);
+        const double Lx = cx[j] * (2. * c - xe_at(x1, x2, x3, ord, r + im) - xe_at(x1, x2, x3, ord, r + ip)) +
+                          cu[j] * (c - xe_at(x1, x2, x3, ord, r + N + i)) + cu[j - 1] * (c - xe_at(x1, x2, x3, ord, r - N + i));
         const double res = f[r + i] - Lx, d = x1[r + i] - x2[r + i];
         const size_t plane = size_t(M) * N, q = size_t(j - 1) * N + i;
         feat[q] = float(res);
@@ -70,12 +74,13 @@ __global__ void k_guess_normalize(size_t plane, double npts, const double *__res
 __global__ void __launch_bounds__(GX *GY) k_guess_target(int N, int M, const double *__restrict__ x,
                                                          const double *__restrict__ x1,
                                                          const double *__restrict__ x2,
+                                                         const double *__restrict__ x3, int ord,
                                                          const double *__restrict__ gsc, float *__restrict__ tgt) {
     const int i = blockIdx.x * GX + threadIdx.x, j = 1 + blockIdx.y * GY + threadIdx.y;
     if (i >= N || j > M) return;
     const size_t a = size_t(j) * N + i;
     const double isd = inv_rms(gsc[1], double(N) * M);
-    tgt[size_t(j - 1) * N + i] = float((x[a] - xe_at(x1, x2, a)) * isd);
+    tgt[size_t(j - 1) * N + i] = float((x[a] - xe_at(x1, x2, x3, ord, a)) * isd);
 }
 
 }  // namespace
@@ -92,7 +97,7 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
     // rows 2-5 of the reduction slots: row 1 may hold the pending ||b||^2
     // partials of the right-hand side
     const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
-    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
+    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, xprev3, xe_order(), pcg64.mg.lev[0].cx,
                        pcg64.mg.lev[0].cu, pcg64.wD, gfeat, rf);
     finalize<4>(rf, nblocks(g), gsc, RED_STORE, 0, st);
     hipLaunchKernelGGL(k_guess_normalize, dim3(unsigned((3 * plane + 255) / 256)), dim3(256), 0, st, plane,
@@ -101,6 +106,11 @@ void Poisson::guess_features(float *dst, hipStream_t st) {
 
 bool Poisson::learned_guess(hipStream_t st) {
     if (!net || nsolves < 2) return false;
+    if (net_skip > 0) {
+        net_skip--;
+        guess_skipped++;
+        return false;
+    }
     const size_t plane = size_t(M) * N;
     if (!gfeat) {
         hipMalloc(reinterpret_cast<void **>(&gfeat), sizeof(float) * 3 * plane);
@@ -113,7 +123,7 @@ bool Poisson::learned_guess(hipStream_t st) {
     // rows 2-5 of the reduction slots: row 1 may hold the pending ||b||^2
     // partials of the right-hand side
     const RedSlot rf{red.partial + 2 * size_t(red.max_blocks), red.max_blocks};
-    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, pcg64.mg.lev[0].cx,
+    hipLaunchKernelGGL(k_guess_features, g, dim3(GX, GY), 0, st, N, M, f, xprev, xprev2, xprev3, xe_order(), pcg64.mg.lev[0].cx,
                        pcg64.mg.lev[0].cu, pcg64.wD, gfeat, rf);
     // the extrapolated guess's residual norm goes to the host as well: the
     // solve keeps the network's guess only if it starts lower
@@ -121,7 +131,7 @@ bool Poisson::learned_guess(hipStream_t st) {
     const double npts = double(plane);
     const NetGeo geo{1, M, N, 1};
     const NetFeatIn fin{gfeat, gsc, npts};
-    const NetApply ap{xprev, xprev2, gsc, pcg64.x, N, npts};
+    const NetApply ap{xprev, xprev2, xprev3, gsc, pcg64.x, N, npts, xe_order()};
     guess_from_net = net->infer_guess(geo, fin, ap, gscratch, st) == VVM_OK;
     return guess_from_net;
 }
@@ -143,6 +153,7 @@ int Poisson::collect_reserve(int cap) {
 void Poisson::collect_features(hipStream_t st) {
     sample_open = false;
     if (!samples || sample_count >= sample_cap || nsolves < 2) return;
+    sample_ord = xe_order();
     guess_features(samples + size_t(sample_count) * 4 * M * N, st);
     sample_open = true;
 }
@@ -150,7 +161,7 @@ void Poisson::collect_features(hipStream_t st) {
 void Poisson::collect_target(hipStream_t st) {
     if (!sample_open) return;
     float *tgt = samples + (size_t(sample_count) * 4 + 3) * M * N;
-    hipLaunchKernelGGL(k_guess_target, ggrid(N, M), dim3(GX, GY), 0, st, N, M, pcg64.x, xprev, xprev2, gsc, tgt);
+    hipLaunchKernelGGL(k_guess_target, ggrid(N, M), dim3(GX, GY), 0, st, N, M, pcg64.x, xprev, xprev2, xprev3, sample_ord, gsc, tgt);
     sample_count++;
     sample_open = false;
 }

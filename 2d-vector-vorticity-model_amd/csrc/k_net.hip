@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(256) k_conv(NetGeo g, const __bf16 *__restrict
 #pragma clang fp contract(off)
                     const float o = acc[q][0] + bias[0];
                     const size_t a = size_t(y + 1) * ap.N + x;
-                    ap.x[a] = (2. * ap.x1[a] - ap.x2[a]) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
+                    ap.x[a] = extrap_at(ap.x1, ap.x2, ap.x3, ap.ord, a) + sqrt(ap.sumsq[1] / ap.npts) * double(o);
                 }
             }
         }

@@ -99,7 +99,8 @@ __global__ void __launch_bounds__(LX *LY) k_pcg_update(int N, int M, T *x, T *r,
 }
 
 // The solve's first residual with the guess or the refinement correction
-// folded in.  XF_EXTRAP: x = 2 x1 - x2 (the extrapolated guess), XF_ZERO:
+// folded in.  XF_EXTRAP: x = the extrapolation of order ord (2 x1 - x2, or
+// 3 (x1 - x2) + x3), XF_ZERO:
 // x = 0, XF_CORR: x = xin + e32 (the FP32 correction; written to another
 // buffer, since neighbouring workgroups read xin), XF_NONE: x as it is.
 // Every value of x the stencil needs is formed on the fly with the same
@@ -110,12 +111,12 @@ enum { XF_NONE = 0, XF_EXTRAP, XF_ZERO, XF_CORR };
 template <int XF>
 __global__ void __launch_bounds__(LX *LY) k_residual_x(int N, int M, const double *f, const double *xin,
                                                        double *xout, const double *x1, const double *x2,
-                                                       const float *e32, double *r, const double *cx,
+                                                       const double *x3, int ord, const float *e32, double *r, const double *cx,
                                                        const double *cu, const double *wD, RedSlot rs, float *r32,
                                                        float *x32) {
     const int i = blockIdx.x * LX + threadIdx.x, j = 1 + blockIdx.y * LY + threadIdx.y;
     auto xv = [&](size_t a) -> double {
-        if (XF == XF_EXTRAP) return 2. * x1[a] - x2[a];
+        if (XF == XF_EXTRAP) return extrap_at(x1, x2, x3, ord, a);
         if (XF == XF_ZERO) return 0.;
         if (XF == XF_CORR) return xin[a] + double(e32[a]);
         return xin[a];
@@ -197,11 +198,10 @@ __global__ void __launch_bounds__(LX *LY) k_calw_back(Grid G, const double *x, d
     store_westdown(w, G, i, k, x[size_t(k - 1) * N + (i - 1)] / rhow[k]);
 }
 
-__global__ void k_extrapolate(size_t n, double *x, double *x1, double *x2) {
+__global__ void k_extrapolate(size_t n, double *x, const double *x1, const double *x2, const double *x3, int ord) {
     const size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const double a = x1[t], b = x2[t];
-    x[t] = 2. * a - b;
+    x[t] = extrap_at(x1, x2, x3, ord, t);
 }
 
 // ------------------------------------------------------------------ cal_u
@@ -355,6 +355,13 @@ __global__ void __launch_bounds__(UW *UC) k_calu_cols(Grid G, StepPtrs S) {
     }
 }
 
+// LDS budget of the one-workgroup coarse kernel for an FP64-sized budget:
+// the FP32 hierarchy takes the levels the FP64 one takes.
+template <typename T>
+size_t coarse_budget(size_t fp64_bytes) {
+    return fp64_bytes * sizeof(T) / sizeof(double);
+}
+
 template <typename T>
 T *dalloc(size_t n) {
     void *p = nullptr;
@@ -458,9 +465,11 @@ bool build_hier(MGHier<T> &H, int N, int M, const std::vector<double> &cx0, cons
             }
         }
     }
-    // the one-workgroup kernel takes every level that fits in 64 KiB of LDS;
-    // the finest levels run as tiled down/up kernels
-    return mg_plan(H, size_t(64) * 1024) == 0;
+    // the one-workgroup kernel takes every level that fits in 64 KiB of LDS
+    // in FP64 (the same levels in FP32: one more level there, 4x the
+    // points, made the one-CU kernel 50% slower than the tiled launch it
+    // saves); the finest levels run as tiled down/up kernels
+    return mg_plan(H, coarse_budget<T>(size_t(64) * 1024)) == 0;
 }
 
 template <typename T>
@@ -653,8 +662,9 @@ int Poisson::init(const Grid &G, const double *rhou, const double *rhow, const R
     f = dalloc<double>(n);
     xprev = dalloc<double>(n);
     xprev2 = dalloc<double>(n);
+    xprev3 = dalloc<double>(n);
     x64B = dalloc<double>(n);
-    if (!f || !xprev || !xprev2 || !x64B) return VVM_ERR_NOMEM;
+    if (!f || !xprev || !xprev2 || !xprev3 || !x64B) return VVM_ERR_NOMEM;
     // pinned, coherent and mapped: k_finalize writes the CG residual norm
     // here directly, and the host reads it after an event
     if (hipHostMalloc(reinterpret_cast<void **>(&hscal), 16 * sizeof(double),
@@ -683,11 +693,11 @@ static void free_pcg(PCG<T> &P) {
 void Poisson::destroy() {
     free_pcg(pcg64);
     if (have32) free_pcg(pcg32);
-    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)x64B, (void *)x32B, (void *)bcg, (void *)bsc,
+    for (void *p : {(void *)f, (void *)xprev, (void *)xprev2, (void *)xprev3, (void *)x64B, (void *)x32B, (void *)bcg, (void *)bsc,
                     (void *)r64, (void *)gfeat,
                     (void *)gsc, (void *)samples})
         if (p) hipFree(p);
-    f = xprev = xprev2 = r64 = x64B = bcg = bsc = nullptr;
+    f = xprev = xprev2 = xprev3 = r64 = x64B = bcg = bsc = nullptr;
     gfeat = samples = x32B = nullptr;
     bn_pending = 0;
     x_pending = 0;
@@ -709,8 +719,8 @@ void Poisson::destroy() {
 
 int Poisson::replan(size_t budget) {
     if (budget > size_t(64) * 1024) return VVM_ERR_ARG;
-    if (mg_plan(pcg64.mg, budget) != 0) return VVM_ERR_NOMEM;
-    if (have32 && mg_plan(pcg32.mg, budget) != 0) return VVM_ERR_NOMEM;
+    if (mg_plan(pcg64.mg, coarse_budget<double>(budget)) != 0) return VVM_ERR_NOMEM;
+    if (have32 && mg_plan(pcg32.mg, coarse_budget<float>(budget)) != 0) return VVM_ERR_NOMEM;
     lds_budget = budget;
     return VVM_OK;
 }
@@ -740,7 +750,8 @@ void Poisson::flush_pending(hipStream_t st) {
     if (x_pending == XF_ZERO)
         hipMemsetAsync(pcg64.x, 0, n * sizeof(double), st);
     else if (x_pending == XF_EXTRAP)
-        hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, n, pcg64.x, xprev, xprev2);
+        hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, n, pcg64.x, xprev, xprev2, xprev3,
+                           x_pending_ord);
     x_pending = XF_NONE;
 }
 
@@ -756,7 +767,7 @@ void Poisson::first_residual(double *r, bool fp32_rhs, hipStream_t st) {
     float *r32 = fp32_rhs ? pcg32.r : nullptr;
     float *x32 = fp32_rhs ? ((xf == XF_CORR) ? x32B : pcg32.x) : nullptr;
 #define VVM_RESIDUAL_X(XF)                                                                                        \
-    hipLaunchKernelGGL((k_residual_x<XF>), g, b, 0, st, N, M, f, D.x, xo, xprev, xprev2, e32, r, D.mg.lev[0].cx, \
+    hipLaunchKernelGGL((k_residual_x<XF>), g, b, 0, st, N, M, f, D.x, xo, xprev, xprev2, xprev3, x_pending_ord, e32, r, D.mg.lev[0].cx, \
                        D.mg.lev[0].cu, D.wD, red, r32, x32)
     switch (xf) {
         case XF_EXTRAP: VVM_RESIDUAL_X(XF_EXTRAP); break;
@@ -797,10 +808,12 @@ void Poisson::set_guess(int mode, hipStream_t st) {
     guess_from_net = false;
     // formed by the solve's first residual kernel (first_residual)
     x_pending = XF_NONE;
-    if (mode == VVM_GUESS_ZERO)
+    if (mode == VVM_GUESS_ZERO) {
         x_pending = XF_ZERO;
-    else if (mode == VVM_GUESS_EXTRAP && nsolves >= 2)
+    } else if ((mode == VVM_GUESS_EXTRAP || mode == VVM_GUESS_EXTRAP2) && nsolves >= 2) {
         x_pending = XF_EXTRAP;
+        x_pending_ord = (mode == VVM_GUESS_EXTRAP2) ? xe_order() : 1;
+    }
     // VVM_GUESS_PREV (and the learned guess, written by its own kernels)
     // start from the current contents of x.
 }
@@ -808,7 +821,10 @@ void Poisson::set_guess(int mode, hipStream_t st) {
 void Poisson::remember_solution(hipStream_t st) {
     flush_pending(st);  // nothing is pending after a solve; a guess set without one is formed now
     const size_t n = size_t(M + 2) * N;
-    std::swap(xprev, xprev2);
+    double *oldest = xprev3;  // x_{t-3} <- x_{t-2} <- x_{t-1} <- x
+    xprev3 = xprev2;
+    xprev2 = xprev;
+    xprev = oldest;
     hipMemcpyAsync(xprev, pcg64.x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
     nsolves++;
 }
@@ -827,12 +843,18 @@ void Poisson::start_solve(double *r, bool fp32_rhs, PCG<T> *pre_vcycle, int *n_r
         *rn2 = hscal[P_RNORM2];
         if (!judge || attempt > 0) return;
         const double rn2_xe = hscal[H_GUESS + 3];
-        if (std::isfinite(*rn2) && *rn2 <= rn2_xe) return;
+        if (std::isfinite(*rn2) && *rn2 <= rn2_xe) {
+            net_backoff = 0;
+            return;
+        }
+        net_backoff = std::min(std::max(1, 2 * net_backoff), kMaxNetSkip);
+        net_skip = net_backoff;
         // the network's guess starts worse than the extrapolation it corrects
         // (or is not finite): start from the extrapolation instead; the first
         // V-cycle queued on the rejected residual is simply overwritten
         guess_fallbacks++;
         x_pending = XF_EXTRAP;
+        x_pending_ord = xe_order();
     }
 }
 

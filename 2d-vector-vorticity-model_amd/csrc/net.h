@@ -42,19 +42,27 @@ void net_gather(int nb, int ch, int cw, int margin, const CropDesc *d, float *fe
 // Inference ends of the network fused with the guess: the first layer
 // reads the three raw FP32 feature planes and scales feature c by
 // 1/RMS = 1/sqrt(sumsq[c]/npts); the last layer writes the guess
-// x = (2 x1 - x2) + sqrt(sumsq[1]/npts) * out straight into the solver's
-// unknown (row y of the network is row y+1 of x, N columns).
+// x = xe + sqrt(sumsq[1]/npts) * out straight into the solver's unknown
+// (row y of the network is row y+1 of x, N columns), xe the extrapolation
+// of the previous solutions (extrap_at).
 struct NetFeatIn {
     const float *feat;
     const double *sumsq;
     double npts;
 };
 struct NetApply {
-    const double *x1, *x2, *sumsq;
+    const double *x1, *x2, *x3, *sumsq;
     double *x;
     int N;
     double npts;
+    int ord;
 };
+
+// Polynomial extrapolation of the last solutions x1 = x_{t-1}, x2, x3 at
+// element a: order 1 is 2 x1 - x2, order 2 is 3 (x1 - x2) + x3.
+__host__ __device__ inline double extrap_at(const double *x1, const double *x2, const double *x3, int ord, size_t a) {
+    return (ord == 2) ? 3. * (x1[a] - x2[a]) + x3[a] : 2. * x1[a] - x2[a];
+}
 void net_conv_first_features(const NetGeo &g, const NetFeatIn &fin, const __bf16 *wpk, const float *bias, __bf16 *A,
                              hipStream_t st);
 void net_conv_last_apply(const NetGeo &g, const __bf16 *in, const __bf16 *wpk, const float *bias, const NetApply &ap,

@@ -55,13 +55,19 @@ class Poisson {
     void back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStream_t st);
 
     // Guess handling: VVM_GUESS_ZERO clears x, VVM_GUESS_EXTRAP forms
-    // 2 x_{t-1} - x_{t-2}; the others keep the current contents of x.
+    // 2 x_{t-1} - x_{t-2}, VVM_GUESS_EXTRAP2 3 (x_{t-1} - x_{t-2}) + x_{t-3}
+    // (the linear form until three solutions exist); the others keep the
+    // current contents of x.
     void set_guess(int mode, hipStream_t st);
     // Record x as the latest solution (for the extrapolated guess).
     void remember_solution(hipStream_t st);
     double *xprev_ptr() const { return xprev; }
     double *xprev2_ptr() const { return xprev2; }
+    double *xprev3_ptr() const { return xprev3; }
     int nsolves = 0;
+    // order of the extrapolation the learned guess corrects and the
+    // recorded samples use: quadratic once three solutions exist
+    int xe_order() const { return nsolves >= 3 ? 2 : 1; }
     // Plain FP64 MG-PCG.  tol: relative true residual ||b-Ax||/||b||.
     SolveResult solve_fp64(double tol, int max_iter, hipStream_t st);
     // FP32 MG-PCG inside FP64 iterative refinement.
@@ -82,8 +88,8 @@ class Poisson {
     int N = 0, M = 0;
 
     // Learned initial guess (k_guess.hip).  With a network attached and two
-    // previous solutions, x = xe + s_d * net(features), xe = 2 x_{t-1} -
-    // x_{t-2}; returns false (nothing written) otherwise.
+    // previous solutions, x = xe + s_d * net(features), xe the extrapolation
+    // of order xe_order(); returns false (nothing written) otherwise.
     GuessNet *net = nullptr;
     bool learned_guess(hipStream_t st);
     // The network's guess was written for the next solve.  That solve keeps
@@ -93,6 +99,12 @@ class Poisson {
     // or accuracy.  guess_fallbacks counts the restarts.
     bool guess_from_net = false;
     int guess_fallbacks = 0;
+    // A network whose guesses keep losing stops costing its inference: after
+    // a restart it sits out net_skip solves, doubling (1, 2, 4, .. 32) while
+    // its guesses keep losing; a kept guess resets the count.
+    static constexpr int kMaxNetSkip = 32;
+    int net_skip = 0, net_backoff = 0;
+    int guess_skipped = 0;
     // Sample recording for training: features before the solve, target
     // (x - xe) / s_d after it.  cap 0 frees the buffer.
     int collect_reserve(int cap);
@@ -101,6 +113,7 @@ class Poisson {
     float *samples = nullptr;               // [cap][4][M][N]: 3 features, target
     int sample_cap = 0, sample_count = 0;
     bool sample_open = false;               // features written, target pending
+    int sample_ord = 1;                     // extrapolation order of the open sample
 
    private:
     void vcycle64(hipStream_t st, bool first_done, double **zcur);
@@ -111,6 +124,7 @@ class Poisson {
     double *f = nullptr;       // rhs of L x = f, (M+2)*N
     double *xprev = nullptr;   // previous solution (for extrapolation)
     double *xprev2 = nullptr;
+    double *xprev3 = nullptr;
     double *r64 = nullptr;     // FP64 residual of the refinement loop
     // pinned host copy of scalars: [P_RNORM2], [P_BNORM2] of the first
     // residual, [8..] the CG loop's, [H_GUESS..H_GUESS+3] the feature pass's
@@ -134,6 +148,7 @@ class Poisson {
     // callers that use red or x before the solve.
     int bn_pending = 0;
     int x_pending = 0;
+    int x_pending_ord = 1;   // extrapolation order of a pending XF_EXTRAP guess
     bool corr_pending = false;
     double *x64B = nullptr;  // the other FP64 solution buffer (the correction writes it)
     double *bcg = nullptr, *bsc = nullptr;  // BiCGSTAB vectors and scalars (allocated on first use)

@@ -260,8 +260,8 @@ void prepare_guess(vvm_dev *h, const StepPtrs &S, hipStream_t st) {
     h->pois.net = h->guess_net;
     const bool learned = (c.guess_mode == VVM_GUESS_LEARNED) && h->pois.learned_guess(st);
     // without a network (or before two solutions exist) the learned mode
-    // starts from the extrapolation
-    if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP : c.guess_mode, st);
+    // starts from the extrapolation the network would correct
+    if (!learned) h->pois.set_guess(c.guess_mode == VVM_GUESS_LEARNED ? VVM_GUESS_EXTRAP2 : c.guess_mode, st);
 }
 
 // prepared: prepare_guess has already run (on the second stream, which the
@@ -667,7 +667,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
     const Grid &G = h->G;
     const vvm_dev_config &c = h->cfg;
     vvm_step_stats acc{};
-    const int fallbacks0 = h->pois.guess_fallbacks;
+    const int fallbacks0 = h->pois.guess_fallbacks, skipped0 = h->pois.guess_skipped;
     int warn = 0;
     const bool timing = (stats != nullptr);
     auto t0 = std::chrono::steady_clock::now();
@@ -748,6 +748,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         }
         acc.solve_iters_total += ss.iters;
         acc.guess_fallbacks = h->pois.guess_fallbacks - fallbacks0;
+        acc.guess_skipped = h->pois.guess_skipped - skipped0;
         acc.solve_iters_last = ss.iters;
         acc.refine_last = ss.refine;
         acc.rel_residual_last = ss.rel_residual;
@@ -1156,10 +1157,11 @@ int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out) {
     int rc = vvm_dev_download(h, &v);
     if (rc) return rc;
     double *gs = nullptr;
-    HIPCHK(h, hipMalloc(&gs, 3 * nu * sizeof(double)));
+    HIPCHK(h, hipMalloc(&gs, 4 * nu * sizeof(double)));
     hipMemcpyAsync(gs, h->pois.x64(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
     hipMemcpyAsync(gs + nu, h->pois.xprev_ptr(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
     hipMemcpyAsync(gs + 2 * nu, h->pois.xprev2_ptr(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+    hipMemcpyAsync(gs + 3 * nu, h->pois.xprev3_ptr(), nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
     const int nsolves = h->pois.nsolves, scap = h->pois.sample_cap;
     h->pois.sample_cap = h->pois.sample_count;  // record nothing while tuning
 
@@ -1192,6 +1194,7 @@ int vvm_dev_tune(vvm_dev_t *h, int steps, vvm_tune_result *out) {
             if (rc) break;
             hipMemcpyAsync(h->pois.xprev_ptr(), gs + nu, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
             hipMemcpyAsync(h->pois.xprev2_ptr(), gs + 2 * nu, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
+            hipMemcpyAsync(h->pois.xprev3_ptr(), gs + 3 * nu, nu * sizeof(double), hipMemcpyDeviceToDevice, h->st);
             h->pois.nsolves = nsolves;
         }
         if (rc < 0) break;

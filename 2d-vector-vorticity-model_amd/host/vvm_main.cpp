@@ -105,7 +105,8 @@ int main(int argc, char **argv) {
         pick(K.s("VVM_SOLVER", "fp64"),
              {{"fp64", VVM_SOLVER_FP64}, {"mixed", VVM_SOLVER_MIXED}, {"bicgstab", VVM_SOLVER_BICGSTAB}}, "VVM_SOLVER");
     config.guess_mode = pick(K.s("VVM_GUESS", "zero"),
-                             {{"zero", VVM_GUESS_ZERO}, {"prev", VVM_GUESS_PREV}, {"extrap", VVM_GUESS_EXTRAP}},
+                             {{"zero", VVM_GUESS_ZERO}, {"prev", VVM_GUESS_PREV}, {"extrap", VVM_GUESS_EXTRAP},
+                              {"extrap2", VVM_GUESS_EXTRAP2}},
                              "VVM_GUESS");
     config.stop_mode =
         pick(K.s("VVM_STOP", "parity"), {{"parity", VVM_STOP_PARITY}, {"rel", VVM_STOP_REL}, {"abs", VVM_STOP_ABS}},

@@ -21,7 +21,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG_DIR, "libvvm_amd.so")
 
 VVM_SOLVER_FP64, VVM_SOLVER_MIXED, VVM_SOLVER_BICGSTAB = 0, 1, 2
-VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED = 0, 1, 2, 3
+VVM_GUESS_ZERO, VVM_GUESS_PREV, VVM_GUESS_EXTRAP, VVM_GUESS_LEARNED, VVM_GUESS_EXTRAP2 = 0, 1, 2, 3, 4
 VVM_STOP_PARITY, VVM_STOP_REL, VVM_STOP_ABS = 0, 1, 2
 VVM_OK, VVM_WARN_NOT_CONVERGED = 0, 1
 KT = {"none": 0, "pzeta": 1, "thermo4": 2, "scale": 3, "kessler": 4, "turb": 5, "diffusion": 6, "mean": 7,
@@ -73,7 +73,8 @@ class VvmStepStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("solve_iters_total", C.c_int32), ("solve_iters_last", C.c_int32),
                 ("refine_last", C.c_int32), ("not_converged", C.c_int32), ("rel_residual_last", C.c_double),
                 ("t_advection", C.c_double), ("t_poisson", C.c_double), ("t_microphysics", C.c_double),
-                ("t_diffusion", C.c_double), ("t_all", C.c_double), ("guess_fallbacks", C.c_int32)]
+                ("t_diffusion", C.c_double), ("t_all", C.c_double), ("guess_fallbacks", C.c_int32),
+                ("guess_skipped", C.c_int32)]
 
 
 class VvmSolveStats(C.Structure):

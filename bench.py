@@ -11,10 +11,15 @@ Three paths are timed:
   fp64_plain     -- FP64 MG-PCG from a zero guess (the reference's guess);
   mixed          -- FP32 MG-PCG inside FP64 refinement from the linear
                     extrapolation of the last two solutions;
-  mixed_learned  -- the same solver from the conv-net guess, whose weights
-                    come from --net or are trained here first (untimed) on
-                    samples the model records from 1024x128 runs.
-`value` is the mixed_learned path's throughput; all three are printed.
+  mixed_quadratic -- the same solver from the quadratic extrapolation of
+                    the last three solutions;
+  mixed_learned  -- the same solver from the conv-net correction of the
+                    quadratic extrapolation, whose weights come from --net or
+                    are trained here first (untimed) on samples the model
+                    records from 1024x128 runs; a guess that starts worse
+                    than the extrapolation is dropped, and the network then
+                    sits out a growing number of solves.
+`value` is the mixed_learned path's throughput; all four are printed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 Prints one JSON line on stdout; progress goes to stderr.
@@ -185,6 +190,92 @@ def cpu_baseline(case: V.Case, state: V.HostState) -> dict | None:
             "sample": f"{n} steps of the 2048x256 workload, C restatement (direct FFT+Thomas solve), 1 thread"}
 
 
+def k3_zeta(cfg, seed: int, rms: float = 1e-2) -> np.ndarray:
+    """A seeded Gaussian random vorticity field with a k^-3 power spectrum
+    (amplitude spectrum k^-1.5, periodic in x), RMS `rms` s^-1 over the
+    interior, zero on rows k = 0, 1 and nz-1, periodic x ghosts; [i][k]."""
+    nx, nz = cfg.nx, cfg.nz
+    ni, nk = nx - 2, nz - 2
+    rng = np.random.default_rng(seed)
+    spec = np.fft.rfft2(rng.normal(size=(ni, nk)))
+    kx = np.fft.fftfreq(ni)[:, None]
+    kz = np.fft.rfftfreq(nk)[None, :]
+    kk = np.sqrt(kx * kx + kz * kz)
+    kk[0, 0] = np.inf
+    z = np.fft.irfft2(spec * kk ** -1.5, s=(ni, nk))
+    zeta = np.zeros((nx, nz))
+    zeta[1:-1, 1:-1] = z
+    zeta[:, :2] = 0.
+    zeta[:, -1] = 0.
+    zeta *= rms / np.sqrt(np.mean(zeta[1:-1, 1:-1] ** 2))
+    zeta[0, :] = zeta[nx - 2, :]
+    zeta[nx - 1, :] = zeta[1, :]
+    return zeta
+
+
+def calw_rhs(cfg, rhow: np.ndarray, zeta: np.ndarray) -> np.ndarray:
+    """cal_w's right-hand side b = rhow^2 (zeta[i+1] - zeta[i]) dx for
+    k = 2..nz-2 in the reference's row order (src/PoissonSolver.cpp:444-450)."""
+    k = np.arange(2, cfg.nz - 1)
+    d = zeta[2:cfg.nx, 2:cfg.nz - 1] - zeta[1:cfg.nx - 1, 2:cfg.nz - 1]
+    return np.ascontiguousarray((rhow[k] ** 2 * d * cfg.dx).T).ravel()
+
+
+def solve_bench(nfields: int = 3) -> dict:
+    """The isolated elliptic solve on seeded k^-3 vorticity fields at
+    2048x256: iterations and device time to ||b - A x|| / ||b|| <= 1e-10 of
+    FP64 MG-PCG and of FP32 MG-PCG in FP64 refinement (both from a zero
+    guess: a lone solve has no previous solutions to extrapolate or feed the
+    network), the mixed solution's error against the FP64 solution iterated
+    until its residual stops falling, and the reference's Eigen BiCGSTAB +
+    ILUT iterations at the same 1e-10 (oracle/_ref, one thread) where that
+    binary is present."""
+    import tempfile
+    from oracle import oracle as O
+    base = V.case("c2")
+    state = V.init_state(base)
+    rhow = state["rhow"]
+    n = (base.cfg.nx - 2) * (base.cfg.nz - 3)
+    rows = []
+    for seed in range(1, nfields + 1):
+        zeta = k3_zeta(base.cfg, seed)
+        b = calw_rhs(base.cfg, rhow, zeta)
+        row = {"seed": seed}
+        xs = {}
+        for name, over in (("fp64_parity", dict(stop_mode=V.VVM_STOP_PARITY)),
+                           ("fp64", dict(stop_mode=V.VVM_STOP_REL, tol_rel=1e-10)),
+                           ("mixed", dict(stop_mode=V.VVM_STOP_REL, tol_rel=1e-10, solver_mode=V.VVM_SOLVER_MIXED))):
+            c = V.case("c2", max_iter=400, **over)
+            with V.Device(c.cfg) as dev:
+                dev.upload(state)
+                x = np.zeros(n)
+                dev.poisson(b, x.copy())  # warm-up
+                x = np.zeros(n)
+                s = dev.poisson(b, x)
+            xs[name] = x
+            row[name] = {"iters": s.iters, "refine": s.refine, "rel_residual": s.rel_residual,
+                         "ms": 1e3 * s.seconds}
+        ref = xs["fp64_parity"]
+        for name in ("fp64", "mixed"):
+            row[name]["rel_err_vs_fp64"] = float(np.linalg.norm(xs[name] - ref) / np.linalg.norm(ref))
+        if O.ref_available():
+            st = state.copy()
+            st["zetap"][:] = zeta
+            with tempfile.TemporaryDirectory() as tmp:
+                pin, pout = os.path.join(tmp, "in.state"), os.path.join(tmp, "out.state")
+                st.save(pin, base.cfg)
+                O.run_ref("solve", "1e-10", pin, pout, timeout=900)
+                r = V.HostState.read_records(pout)
+            row["reference_bicgstab_ilut"] = {"iters": int(r["solve_iterations"][0]),
+                                              "spmv_equivalents": 2 * int(r["solve_iterations"][0]),
+                                              "rel_residual": float(r["solve_rel_residual"][0]),
+                                              "seconds_1_thread": float(r["wall_seconds"][0])}
+        log(f"[bench] solve seed {seed}: {row}")
+        rows.append(row)
+    return {"workload": "isolated cal_w solve, 2048x256, k^-3 Gaussian zeta fields, RMS 1e-2 s^-1, zero guess, "
+                        "stop at ||b - Ax||/||b|| <= 1e-10", "fields": rows}
+
+
 def copy_calibration(nbytes: int = 256 << 20, reps: int = 3) -> None:
     """A device-to-device copy of known size (the runtime's blit kernel), so
     that FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes."""
@@ -264,6 +355,8 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="time the 64-member ensemble of uneven grids instead of the 2048x256 step")
     ap.add_argument("--net", default=None, help="trained guess network (vvm_net_save file) for the mixed path")
+    ap.add_argument("--solve", action="store_true",
+                    help="also run the isolated elliptic solve on k^-3 vorticity fields (extra 'solve' key)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 256 MiB device-to-device copy (counter calibration)")
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,7 +364,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel", default="diffusion", choices=sorted(KERNEL_BYTES_PER_POINT))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--paths", default="fp64_plain,mixed,mixed_learned")
+    ap.add_argument("--paths", default="fp64_plain,mixed,mixed_quadratic,mixed_learned")
     ap.add_argument("--train-steps", type=int, default=6000,
                     help="Adam steps of the in-process training of the guess network (untimed)")
     args = ap.parse_args()
@@ -286,6 +379,7 @@ def main():
     paths = {
         "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
         "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+        "mixed_quadratic": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP2),
         "mixed_learned": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_LEARNED),
     }
     wanted = [p for p in args.paths.split(",") if p]
@@ -309,6 +403,7 @@ def main():
             "solve_iters_per_step": stats.solve_iters_total / max(args.steps, 1),
             "refine_last": stats.refine_last, "rel_residual_last": stats.rel_residual_last,
             "not_converged": stats.not_converged,
+            "guess_fallbacks": stats.guess_fallbacks, "guess_skipped": stats.guess_skipped,
             "phase_ms_last_step": {"advection": 1e3 * stats.t_advection, "poisson": 1e3 * stats.t_poisson,
                                    "microphysics": 1e3 * stats.t_microphysics,
                                    "diffusion": 1e3 * stats.t_diffusion},
@@ -338,7 +433,7 @@ def main():
                      "bytes_per_launch": kbytes, "avg_launch_us": R["kernel_avg_us"],
                      "traffic": measured_traffic(args.kernel), "traffic_source": os.path.relpath(TRAFFIC_TABLE, ROOT)},
     }
-    for p in ("mixed", "mixed_learned"):
+    for p in ("mixed", "mixed_quadratic", "mixed_learned"):
         if "fp64_plain" in results and p in results:
             out[f"speedup_{p}_vs_fp64"] = results[p]["value"] / results["fp64_plain"]["value"]
             a, b = finals[p], finals["fp64_plain"]
@@ -346,6 +441,8 @@ def main():
                                           for f in ("zeta", "th", "qv", "qc", "qr", "u", "w")}
     if "mixed_learned" in results:
         out["learned_guess"] = learned_info
+    if args.solve:
+        out["solve"] = solve_bench()
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (one reference step at 2048x256, single thread)")
         cb = cpu_baseline(base, state)

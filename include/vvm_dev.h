@@ -54,7 +54,10 @@ extern "C" {
 #define VVM_GUESS_ZERO 0     /* reference behaviour (Eigen x.setZero())     */
 #define VVM_GUESS_PREV 1     /* x from the previous step                    */
 #define VVM_GUESS_EXTRAP 2   /* 2 x_{t-1} - x_{t-2}                         */
-#define VVM_GUESS_LEARNED 3  /* conv-net correction on top of x_{t-1}       */
+#define VVM_GUESS_LEARNED 3  /* conv-net correction of the quadratic
+                                extrapolation (VVM_GUESS_EXTRAP2)             */
+#define VVM_GUESS_EXTRAP2 4  /* 3 (x_{t-1} - x_{t-2}) + x_{t-3}; the linear
+                                form until three solutions exist             */
 
 /* Stop rules of the elliptic solve. */
 #define VVM_STOP_PARITY 0  /* iterate until the residual stops decreasing   */
@@ -122,6 +125,10 @@ typedef struct vvm_step_stats {
     int32_t guess_fallbacks;   /* solves whose learned guess started worse
                                   than the extrapolation (or not finite) and
                                   were restarted from the extrapolation    */
+    int32_t guess_skipped;     /* solves that did not run the network: after
+                                  a restart it sits out 1, 2, 4, .. 32
+                                  solves (doubling while its guesses keep
+                                  losing, reset by a kept guess)            */
 } vvm_step_stats;
 
 typedef struct vvm_solve_stats {

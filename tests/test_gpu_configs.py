@@ -10,7 +10,8 @@ branches the benchmark cases leave off, against the CPU oracle.
   - config 2 (2048 x 256, dx = dz = 100 m, dt = 1 s): two FP64 steps at the
     parity stop rule against the oracle (1e-10 per field); 20 steps of each
     benchmark path (FP64 from the zero guess, mixed precision from the
-    extrapolated guess, mixed precision from the learned guess), every solve
+    linear and the quadratic extrapolation, mixed precision from the learned
+    guess), every solve
     at ||b - Ax|| / ||b|| <= 1e-10 and every field within 1e-6 relative L2
     of the FP64 oracle.
   - config 4: the bench's 64-member draw of uneven grids (nx in {256, 512,
@@ -117,6 +118,7 @@ def test_c2_two_steps_parity(c2_oracle):
 C2_PATHS = {
     "fp64_plain": dict(solver_mode=V.VVM_SOLVER_FP64, guess_mode=V.VVM_GUESS_ZERO),
     "mixed": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP),
+    "mixed_quadratic": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_EXTRAP2),
     "mixed_learned": dict(solver_mode=V.VVM_SOLVER_MIXED, guess_mode=V.VVM_GUESS_LEARNED),
 }
 
