@@ -423,7 +423,7 @@ def main():
         "metric": "grid-point-steps/s at 2048x256, FP64 plain vs mixed+learned; Poisson iters to 1e-10",
         "value": R["value"], "unit": "grid-point-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": R["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "config 2: 2048x256 interior, dx=dz=100 m, dt=1 s, Kessler, sigma=0.1 K theta "
                                "noise < 500 m, 100 W/m^2 surface heat flux, qv surface ramp; solve to 1e-10",
                    "grid_points": pts, "path": head},
