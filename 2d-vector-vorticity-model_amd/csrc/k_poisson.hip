@@ -804,7 +804,6 @@ void Poisson::back_to_w(const Grid &G, double *w, const double *rhow_dev, hipStr
 }
 
 void Poisson::set_guess(int mode, hipStream_t st) {
-    (void)st;
     guess_from_net = false;
     // formed by the solve's first residual kernel (first_residual)
     x_pending = XF_NONE;
@@ -814,18 +813,25 @@ void Poisson::set_guess(int mode, hipStream_t st) {
         x_pending = XF_EXTRAP;
         x_pending_ord = (mode == VVM_GUESS_EXTRAP2) ? xe_order() : 1;
     }
-    // VVM_GUESS_PREV (and the learned guess, written by its own kernels)
-    // start from the current contents of x.
+    // VVM_GUESS_PREV, and the extrapolations before two solutions exist,
+    // start from the last solution (x holds the oldest one after
+    // remember_solution); the learned guess is written by its own kernels.
+    if (x_pending == XF_NONE && mode != VVM_GUESS_ZERO && nsolves >= 1)
+        hipMemcpyAsync(pcg64.x, xprev, size_t(M + 2) * N * sizeof(double), hipMemcpyDeviceToDevice, st);
 }
 
 void Poisson::remember_solution(hipStream_t st) {
     flush_pending(st);  // nothing is pending after a solve; a guess set without one is formed now
     const size_t n = size_t(M + 2) * N;
-    double *oldest = xprev3;  // x_{t-3} <- x_{t-2} <- x_{t-1} <- x
+    (void)n;
+    // x_{t-3} <- x_{t-2} <- x_{t-1} <- x by rotating the buffers, no copy:
+    // the oldest becomes the next solve's x, which set_guess fills (or the
+    // first residual kernel, or the network, rewrites in full)
+    double *oldest = xprev3;
     xprev3 = xprev2;
     xprev2 = xprev;
-    xprev = oldest;
-    hipMemcpyAsync(xprev, pcg64.x, n * sizeof(double), hipMemcpyDeviceToDevice, st);
+    xprev = pcg64.x;
+    pcg64.x = oldest;
     nsolves++;
 }
 

@@ -282,8 +282,8 @@ int solve_w(vvm_dev *h, const StepPtrs &S, vvm_solve_stats *ss, bool prepared = 
         R = h->pois.solve_fp64(tol, maxit, h->st);
     }
     h->pois.collect_target(h->st);  // before x_{t-1}, x_{t-2} move on
-    h->pois.remember_solution(h->st);
     h->pois.back_to_w(h->G, S.w, h->P.rhow, h->st);
+    h->pois.remember_solution(h->st);  // x becomes x_{t-1} (buffer rotation)
     if (ss) {
         ss->iters = R.iters;
         ss->refine = R.refine;

@@ -303,7 +303,7 @@ def ensemble_bench(steps: int, net_path: str | None, seed: int = 4) -> dict:
     rng = np.random.default_rng(seed)
     draws = [(int(rng.choice([256, 512, 1024, 2048])), int(rng.choice([64, 128, 256])),
               float(rng.uniform(0.05, 0.2)), float(rng.uniform(50, 200))) for _ in range(64)]
-    guess = V.VVM_GUESS_LEARNED if net_path else V.VVM_GUESS_EXTRAP
+    guess = V.VVM_GUESS_LEARNED if net_path else V.VVM_GUESS_EXTRAP2
 
     def cases(**over):
         return [V.grid_case(nx, nz, seed=1000 + k, sigma=sg, heat=hf, **over)
@@ -344,7 +344,7 @@ def ensemble_bench(steps: int, net_path: str | None, seed: int = 4) -> dict:
         "higher_is_better": True, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": "config 4: 64 members, nx in {256,512,1024,2048}, nz in {64,128,256}, "
                                "dx=dz=100 m", "grid_points_total": pts,
-                   "guess": "learned" if net_path else "extrapolated"},
+                   "guess": "learned" if net_path else "quadratic extrapolation"},
         "fp64_value": out["fp64"]["value"], "paths": out,
         "member_error_max_rel_l2": max(errs), "member_error_median_rel_l2": float(np.median(errs)),
     }
@@ -364,6 +364,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--kernel", default="diffusion", choices=sorted(KERNEL_BYTES_PER_POINT))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=30,
+                    help="steps of the 64-member ensemble reported beside the 2048x256 paths (0: skip)")
     ap.add_argument("--paths", default="fp64_plain,mixed,mixed_quadratic,mixed_learned")
     ap.add_argument("--train-steps", type=int, default=6000,
                     help="Adam steps of the in-process training of the guess network (untimed)")
@@ -443,6 +445,11 @@ def main():
         out["learned_guess"] = learned_info
     if args.solve:
         out["solve"] = solve_bench()
+    if args.ensemble_steps > 0:
+        e = ensemble_bench(args.ensemble_steps, args.net)
+        out["ensemble"] = {k: e[k] for k in ("metric", "value", "unit", "steps", "fp64_value",
+                                             "member_error_max_rel_l2", "member_error_median_rel_l2")}
+        out["ensemble"]["config"] = e["config"]
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (one reference step at 2048x256, single thread)")
         cb = cpu_baseline(base, state)
