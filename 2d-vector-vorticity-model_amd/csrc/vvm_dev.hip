@@ -75,6 +75,11 @@ struct vvm_dev {
     bool rad_on = false;
     int radcool = 0;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_mp[2] = {};
+    // per-step phase timers (vvm_dev_set_step_timers): 8 events a step (the
+    // six phase boundaries of ev, then the two of ev_mp) and where the times go
+    std::vector<hipEvent_t> tev;
+    double *tout = nullptr;
+    int tcap = 0;
     // the learned guess (right-hand side, features, network) on the second
     // stream beside the scalar advection: forked after pzeta, joined before
     // the solve
@@ -416,12 +421,28 @@ void vvm_dev_destroy(vvm_dev_t *h) {
         if (e) hipEventDestroy(e);
     for (hipEvent_t e : {h->ev_fork0, h->ev_guess, h->ev_fork, h->ev_join, h->ev_mp[0], h->ev_mp[1]})
         if (e) hipEventDestroy(e);
+    for (hipEvent_t e : h->tev)
+        if (e) hipEventDestroy(e);
     if (h->side) hipStreamDestroy(h->side);
     if (h->st) hipStreamDestroy(h->st);
     delete h;
 }
 
 const char *vvm_dev_last_error(const vvm_dev_t *h) { return h ? h->err.c_str() : "null handle"; }
+
+int vvm_dev_set_step_timers(vvm_dev_t *h, double *out, int cap) {
+    if (!h || cap < 0 || (cap > 0 && !out)) return VVM_ERR_ARG;
+    HIPCHK(h, hipSetDevice(h->device));
+    const size_t need = size_t(out ? cap : 0) * 8;
+    while (h->tev.size() < need) {
+        hipEvent_t e = nullptr;
+        HIPCHK(h, hipEventCreate(&e));
+        h->tev.push_back(e);
+    }
+    h->tout = out;
+    h->tcap = out ? cap : 0;
+    return VVM_OK;
+}
 
 int vvm_dev_upload(vvm_dev_t *h, const vvm_host_view *v) {
     if (!h || !v) return VVM_ERR_ARG;
@@ -680,6 +701,12 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         }
         StepPtrs S = ptrs(h);
         const bool last = (n == nsteps - 1) && timing;
+        const bool per_step = n < h->tcap;
+        // phase boundary `slot` (0-5: ev, 6-7: ev_mp) of this step, on stream s
+        auto rec = [&](int slot, hipStream_t s) {
+            if (per_step) hipEventRecord(h->tev[size_t(n) * 8 + slot], s);
+            if (last) hipEventRecord(slot < 6 ? h->ev[slot] : h->ev_mp[slot - 6], s);
+        };
         if (h->overlap_c && !h->mp_fn && !h->side)
             HIPCHK(h, hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
         const bool overlap = h->overlap_c && !h->mp_fn;
@@ -687,7 +714,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
         // solve; it needs only pzeta's zetap and the past solutions
         const bool early_guess = overlap && c.guess_mode == VVM_GUESS_LEARNED && h->guess_net &&
                                  h->pois.sample_cap == 0;
-        if (last) hipEventRecord(h->ev[0], h->st);
+        rec(0, h->st);
         // A: advection, buoyancy, forcing, halos (src/Iteration.cpp:263-285)
         {
             KTimer t(h, VVM_KT_PZETA);
@@ -711,7 +738,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             launch_bc_uw(G, S, h->st);
             h->ghosts_dirty = false;
         }
-        if (last) hipEventRecord(h->ev[1], h->st);
+        rec(1, h->st);
         // C: Kessler + NegativeValueProcess x3 + halos (:303-311, 404), on
         // the side stream while B runs; the main stream waits for it before D
         // (with host microphysics, C runs after B on the host instead)
@@ -721,7 +748,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             hipStreamWaitEvent(h->side, h->ev_fork, 0);
         }
         if (!h->mp_fn) {
-            if (last) hipEventRecord(h->ev_mp[0], cs);
+            rec(6, cs);
             {
                 KTimer t(h, VVM_KT_KESSLER, cs);
                 launch_kessler(G, S, h->red.negval, cs);
@@ -730,7 +757,7 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
                 KTimer t(h, VVM_KT_SCALE, cs);
                 launch_scale_forcing(G, S, false, false, false, cs);
             }
-            if (last) hipEventRecord(h->ev_mp[1], cs);
+            rec(7, cs);
         }
         if (overlap) hipEventRecord(h->ev_join, h->side);
         // B: pubarTop_pt, cal_w, cal_u (:292-296)
@@ -756,15 +783,15 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             KTimer t(h, VVM_KT_CALU);
             launch_cal_u(G, S, h->st);
         }
-        if (last) hipEventRecord(h->ev[2], h->st);
+        rec(2, h->st);
         if (overlap) hipStreamWaitEvent(h->st, h->ev_join, 0);
         if (h->mp_fn) {
-            if (last) hipEventRecord(h->ev_mp[0], h->st);
+            rec(6, h->st);
             const int mrc = host_microphysics(h, S);
             if (mrc) return mrc;
-            if (last) hipEventRecord(h->ev_mp[1], h->st);
+            rec(7, h->st);
         }
-        if (last) hipEventRecord(h->ev[3], h->st);
+        rec(3, h->st);
         // D: RKM_RKH with Mparam/Hparam, nudging (:408-417); the first
         // updateMean (:408) is overwritten by nextTimeStep's before any read.
         {
@@ -776,14 +803,14 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
             launch_diffusion(G, S, 1 | (c.CASE != 2 ? 2 : 0), h->st);
         }
         if (c.moisture_nudge_time != 0) launch_nudge_qv(G, S, c.moisture_nudge_time, h->st);
-        if (last) hipEventRecord(h->ev[4], h->st);
+        rec(4, h->st);
         // E: nextTimeStep (:153-204)
         {
             KTimer t(h, VVM_KT_MEAN);
             launch_update_mean(G, S, true, h->st);
         }
         rotate(h);
-        if (last) hipEventRecord(h->ev[5], h->st);
+        rec(5, h->st);
         int lrc = check_launch(h);
         if (lrc) return lrc;
         h->stepped = true;
@@ -794,6 +821,20 @@ int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats) {
     double derr = 0.;
     HIPCHK(h, hipMemcpy(&derr, h->s + S_DIFF_ERR, sizeof(double), hipMemcpyDeviceToHost));
     if (derr != 0.) return fail(h, VVM_ERR_STATE, "diffusion sweep: a wavefront gave up waiting for its carry");
+    // per-step phase times, as the reference fills t_*[step] every step
+    for (int n = 0; n < std::min(nsteps, h->tcap); n++) {
+        const hipEvent_t *E = &h->tev[size_t(n) * 8];
+        auto span = [&](hipEvent_t a, hipEvent_t b) {
+            float ms = 0.f;
+            return hipEventElapsedTime(&ms, a, b) == hipSuccess ? double(ms) * 1e-3 : 0.;
+        };
+        double *o = h->tout + size_t(n) * 5;
+        o[0] = span(E[0], E[1]);  // advection (phase A)
+        o[1] = span(E[1], E[2]);  // poisson (phase B)
+        o[2] = span(E[6], E[7]);  // microphysics (phase C, possibly on the second stream)
+        o[3] = span(E[3], E[4]);  // diffusion and nudging (phase E)
+        o[4] = span(E[0], E[5]);  // the whole step on the device
+    }
     if (stats) {
         *stats = acc;
         stats->steps = nsteps;

@@ -928,8 +928,11 @@ void vvm::Iteration::TimeMarching(vvm &model) {
             next++;
         const int n = next - model.step;
         vvm_step_stats st{};
+        std::vector<double> times(size_t(n) * 5, 0.);
+        check(model, vvm_dev_set_step_timers(model.dev, times.data(), n), "vvm_dev_set_step_timers");
         int rc = vvm_dev_step(model.dev, n, &st);
         check(model, rc, "vvm_dev_step");
+        check(model, vvm_dev_set_step_timers(model.dev, nullptr, 0), "vvm_dev_set_step_timers");
         if (st.not_converged > 0)
             std::printf("The Poisson solver did not converge in %d of %d steps (last residual %.3e)\n",
                         st.not_converged, n, st.rel_residual_last);
@@ -938,12 +941,14 @@ void vvm::Iteration::TimeMarching(vvm &model) {
         const int T = model.TIMEROUTPUTSIZE > 0 ? model.TIMEROUTPUTSIZE : 1;
         for (int s = model.step; s < next; s++) {
             if (model.print_steps) std::cout << s << "\n";
+            // each step's own phase times (device events of that step)
             const int slot = s % T;
-            model.t_all[slot] = st.t_all;
-            model.t_advection[slot] = st.t_advection;
-            model.t_poisson[slot] = st.t_poisson;
-            model.t_microphysics[slot] = st.t_microphysics;
-            model.t_diffusion[slot] = st.t_diffusion;
+            const double *t = &times[size_t(s - model.step) * 5];
+            model.t_advection[slot] = t[0];
+            model.t_poisson[slot] = t[1];
+            model.t_microphysics[slot] = t[2];
+            model.t_diffusion[slot] = t[3];
+            model.t_all[slot] = t[4];
         }
         model.step = next;
         if (radiation && next % model.radiation_every == 0) {

@@ -124,6 +124,7 @@ def load_lib(path: str = LIB_PATH):
         "vvm_dev_upload": (C.c_int, [C.c_void_p, P(VvmHostView)]),
         "vvm_dev_download": (C.c_int, [C.c_void_p, P(VvmHostView)]),
         "vvm_dev_step": (C.c_int, [C.c_void_p, C.c_int, P(VvmStepStats)]),
+        "vvm_dev_set_step_timers": (C.c_int, [C.c_void_p, PTR, C.c_int]),
         "vvm_dev_poisson": (C.c_int, [C.c_void_p, PTR, PTR, P(VvmSolveStats)]),
         "vvm_dev_poisson_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, P(VvmSolveStats)]),
         "vvm_dev_op": (C.c_int, [C.c_void_p, C.c_int]),
@@ -429,6 +430,17 @@ class Device:
         s = VvmStepStats()
         self._check(self.lib.vvm_dev_step(self.h, n, C.byref(s)), "step")
         return s
+
+    def step_timed(self, n: int) -> tuple[VvmStepStats, np.ndarray]:
+        """n steps with every step's phase times: an (n, 5) array of seconds,
+        columns advection, poisson, microphysics, diffusion, all."""
+        t = np.zeros((n, 5))
+        self._check(self.lib.vvm_dev_set_step_timers(self.h, t.ctypes.data_as(PTR), n), "set_step_timers")
+        try:
+            s = self.step(n)
+        finally:
+            self.lib.vvm_dev_set_step_timers(self.h, None, 0)
+        return s, t
 
     def kernel_timer(self, kernel: int) -> None:
         self._check(self.lib.vvm_dev_kernel_timer(self.h, kernel), "kernel_timer")

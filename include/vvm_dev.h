@@ -158,6 +158,13 @@ int vvm_dev_download(vvm_dev_t *h, vvm_host_view *v);
  * (src/Iteration.cpp:227-436) on the device; returns after the stream
  * synchronises.  stats may be NULL. */
 int vvm_dev_step(vvm_dev_t *h, int nsteps, vvm_step_stats *stats);
+/* Per-step phase timers, as the reference records t_advection, t_poisson,
+ * t_microphysics, t_diffusion and t_all for every step
+ * (src/Iteration.cpp:286,297,405,418,435): with out set, each later
+ * vvm_dev_step writes the device times (s) of its steps s = 0 .. min(nsteps,
+ * cap) - 1 to out[5 s + 0..4] in that order (t_all: the whole step on the
+ * device).  out must hold 5 * cap doubles; NULL (or cap 0) turns it off. */
+int vvm_dev_set_step_timers(vvm_dev_t *h, double *out, int cap);
 
 /* Isolated elliptic solve of the reference's cal_w system A x = b
  * (src/PoissonSolver.cpp:441-467, matrix src/PoissonSolver.cpp:760-796):

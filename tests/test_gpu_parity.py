@@ -360,3 +360,27 @@ def test_single_operator_parity(oracle, name):
         else:
             e = rel_err(got[f], want[f])
             assert e <= 1e-12, f"{name}/{f}: rel {e:.3e}"
+
+
+def test_per_step_phase_timers():
+    """Every step of a batch gets its own phase times (the reference fills
+    t_*[step] each step, src/Iteration.cpp:286,297,405,418,435): positive,
+    each phase within the step's total, the last step's equal to the batch
+    statistics' (the same boundaries), and the steps' values not all equal."""
+    c = V.case("c0", stop_mode=V.VVM_STOP_REL, tol_rel=1e-10)
+    print("per-step timers: 6 steps of c0", flush=True)
+    with V.Device(c.cfg) as dev:
+        dev.upload(V.init_state(c))
+        s, t = dev.step_timed(6)
+        s2 = dev.step(2)
+    assert s.steps == 6 and t.shape == (6, 5)
+    assert np.all(t > 0), t
+    for col in (0, 1, 3):
+        assert np.all(t[:, col] <= t[:, 4] * 1.0001), t
+    # the per-step and the batch events of a boundary are two records made
+    # one after the other (measured ~1 us apart), against phases of 30 us
+    # and more
+    assert abs(t[-1, 0] - s.t_advection) <= 5e-6 and abs(t[-1, 1] - s.t_poisson) <= 5e-6
+    assert abs(t[-1, 3] - s.t_diffusion) <= 5e-6 and abs(t[-1, 2] - s.t_microphysics) <= 5e-6
+    assert len(set(np.round(t[:, 4], 9))) > 1
+    assert s2.steps == 2
