@@ -853,7 +853,7 @@ void Poisson::start_solve(double *r, bool fp32_rhs, PCG<T> *pre_vcycle, int *n_r
             net_backoff = 0;
             return;
         }
-        net_backoff = std::min(std::max(1, 2 * net_backoff), kMaxNetSkip);
+        net_backoff = std::min(std::max(1, 4 * net_backoff), kMaxNetSkip);
         net_skip = net_backoff;
         // the network's guess starts worse than the extrapolation it corrects
         // (or is not finite): start from the extrapolation instead; the first

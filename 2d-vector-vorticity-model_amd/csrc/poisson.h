@@ -100,9 +100,9 @@ class Poisson {
     bool guess_from_net = false;
     int guess_fallbacks = 0;
     // A network whose guesses keep losing stops costing its inference: after
-    // a restart it sits out net_skip solves, doubling (1, 2, 4, .. 32) while
-    // its guesses keep losing; a kept guess resets the count.
-    static constexpr int kMaxNetSkip = 32;
+    // a restart it sits out net_skip solves, 1, 4, 16, then 64 while its
+    // guesses keep losing; a kept guess resets the count.
+    static constexpr int kMaxNetSkip = 64;
     int net_skip = 0, net_backoff = 0;
     int guess_skipped = 0;
     // Sample recording for training: features before the solve, target

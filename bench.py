@@ -18,7 +18,7 @@ Three paths are timed:
                     are trained here first (untimed) on samples the model
                     records from 1024x128 runs; a guess that starts worse
                     than the extrapolation is dropped, and the network then
-                    sits out a growing number of solves.
+                    sits out 1, 4, 16, then 64 solves.
 `value` is the mixed_learned path's throughput; all four are printed.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]

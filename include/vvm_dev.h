@@ -126,9 +126,9 @@ typedef struct vvm_step_stats {
                                   than the extrapolation (or not finite) and
                                   were restarted from the extrapolation    */
     int32_t guess_skipped;     /* solves that did not run the network: after
-                                  a restart it sits out 1, 2, 4, .. 32
-                                  solves (doubling while its guesses keep
-                                  losing, reset by a kept guess)            */
+                                  a restart it sits out 1, 4, 16, then 64
+                                  solves (while its guesses keep losing;
+                                  reset by a kept guess)                     */
 } vvm_step_stats;
 
 typedef struct vvm_solve_stats {

@@ -84,8 +84,8 @@ def test_bad_learned_guess_falls_back_to_extrapolation(kind):
     want, s_ex = _run(None)
     got, s_le = _run(p)
     # the learned guess needs two previous solutions: solves 3..10 may use
-    # it; after each restart the network sits out 1, 2, 4 .. solves
-    # (solves 3, 5, 8 run it and are restarted; 4, 6-7, 9-10 skip it)
+    # it; after each restart the network sits out 1, 4, 16 .. solves
+    # (solves 3, 5, 10 run it and are restarted; 4, 6-9 skip it)
     assert s_le.guess_fallbacks == 3, s_le.guess_fallbacks
     assert s_le.guess_skipped == 5, s_le.guess_skipped
     assert s_le.not_converged == 0 and s_le.rel_residual_last <= 1e-10
