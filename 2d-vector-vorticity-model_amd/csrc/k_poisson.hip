@@ -822,8 +822,6 @@ void Poisson::set_guess(int mode, hipStream_t st) {
 
 void Poisson::remember_solution(hipStream_t st) {
     flush_pending(st);  // nothing is pending after a solve; a guess set without one is formed now
-    const size_t n = size_t(M + 2) * N;
-    (void)n;
     // x_{t-3} <- x_{t-2} <- x_{t-1} <- x by rotating the buffers, no copy:
     // the oldest becomes the next solve's x, which set_guess fills (or the
     // first residual kernel, or the network, rewrites in full)
